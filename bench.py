@@ -1,0 +1,182 @@
+"""Benchmark of the DBSCAN hot path on one MI355X.
+
+A step is one full pass of the path over the resident input: classify
+(neighbour counts + core flags) and label (union-find + border attachment),
+ending with the (cores, noise, clusters) triple on the host; coordinates are
+in HBM before the timed region starts and per-point results stay in HBM.
+
+    python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline]
+
+Prints one JSON line.  `value` is points/s = N / step time.  `roofline` is
+computed from HIP events recorded on the library's stream around every
+launch of the dominant kernel in the timed steps.  `cpu_baseline` times the
+reference's own OpenMP process_points (oracle/_ref/ref_harness) on a bounded
+sample of chunk rows of the same input and extrapolates to all N*N pair tests
+of its Classify phase; its Assign-Labels phase is not included, so the CPU
+figure is an upper bound on the reference's points/s.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "nextdbscan-exa_amd"))
+
+METRIC = "points/s clustered, plain vs MFMA-filter/batched, exact labels; % HBM/MFMA peak"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+DEFAULT_WORKLOAD = "blobs3d_10m"
+
+
+def algorithmic_bytes(kernel, n, d, cells):
+    """Bytes one launch must move at minimum (reads of its inputs, writes of
+    its outputs), per kernel: N points of d float32, C_ne non-empty cells."""
+    table = {
+        "count": n * (4 * d + 4) + 8 * cells,          # sorted coords in, nn out, cell table
+        "union": n * (4 * d + 1) + 8 * cells,          # coords + core flags in, parents
+        "border": n * (4 * d + 1 + 4) + 8 * cells,     # coords + core flags in, labels out
+        "radix_scatter": n * 24,                        # key+val in and out
+        "brute_count": n * (4 * d + 4),
+    }
+    return table.get(kernel)
+
+
+def cpu_baseline(x, eps_text, m, n):
+    """Reference OpenMP Classify on sampled chunk rows (oracle/_ref)."""
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.path.exists(harness):
+        return None
+    threads = max(1, min(16, os.cpu_count() or 1))
+    rows = max(64, min(n, int(2.5e10 / max(n, 1))))
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "in.bin")
+        import synth
+        synth.write_bin(p, x)
+        out = subprocess.run([harness, "classify_sample", p, str(m), eps_text, str(threads), str(rows), "7"],
+                             capture_output=True, text=True, timeout=600, check=True,
+                             env=dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores"))
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    full_ms = r["ms"] * (n / r["rows"])
+    return {
+        "value": n / (full_ms * 1e-3),
+        "unit": "points/s",
+        "cores": threads,
+        "kind": "reference",
+        "sample": (f"reference process_points (data_process.cpp:395-458, OpenMP, {threads} threads) over "
+                   f"{r['rows']} random chunk rows x {n} points = {r['pair_tests']:.3g} pair tests in "
+                   f"{r['ms'] / 1e3:.1f} s, extrapolated to the Classify phase's {n}x{n} tests "
+                   f"({full_ms / 1e3:.4g} s); Assign Labels (>= one more N x N_core sweep) excluded, "
+                   f"so this is an upper bound on the reference's points/s"),
+        "pair_tests_per_s": r["tests_per_s"],
+    }
+
+
+def traffic_from_profiles(kernel, workload):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration), when a committed
+    summary for this kernel and workload exists."""
+    p = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD)
+    ap.add_argument("--n", type=int, default=None, help="smaller instance of the same workload shape")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs one GPU: every box in this setup has one MI355X, and the multi-GPU "
+              "(RCCL/xGMI slab sharding) path is a design note, not built.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import ndbx
+    import synth
+
+    if not ndbx.device_available():
+        raise RuntimeError("no HIP device")
+    x, eps, m = synth.make(args.workload, n=args.n)
+    n, d = x.shape
+    eps_text = {"blobs2d_100k": "0.5", "blobs3d_10m": "1.5", "skew2d_100m": "0.3"}.get(args.workload, repr(float(eps)))
+
+    ctx = ndbx.Context(x, eps, m, flags=ndbx.PROFILE)
+    for _ in range(args.warmup):
+        ctx.classify().label()
+        ctx.meta()
+    ctx.profile(reset=True)
+    torch.cuda.synchronize()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.classify().label()
+        meta = ctx.meta()
+    ctx.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    prof = ctx.profile()
+    stats = ctx.stats()
+    path = ctx.path()
+    nn = ctx.nn()
+    ctx.close()
+
+    kernels = {k: {"ms_avg": v[0] / max(v[1], 1), "launches_per_step": v[1] / args.steps}
+               for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
+    dominant = max(prof.items(), key=lambda kv: kv[1][0])[0] if prof else None
+    roofline = None
+    if dominant:
+        total_ms, launches = prof[dominant]
+        avg_s = total_ms / launches * 1e-3
+        ab = algorithmic_bytes(dominant, n, d, stats["cells"])
+        if ab is not None:
+            achieved = ab / avg_s / 1e9
+            roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": traffic_from_profiles(dominant, args.workload),
+                        "algorithmic_bytes": ab, "avg_ms": avg_s * 1e3}
+            if dominant in ("count", "brute_count"):
+                # compute view: F = 3d flops per true (ordered) neighbour pair
+                flops = 3.0 * d * float(np.asarray(nn, np.int64).sum())
+                tf = flops / avg_s / 1e12
+                roofline["valu"] = {"achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                    "frac": tf / FP32_PEAK_TFLOPS, "flops": flops}
+    cpu = None if args.no_cpu_baseline else cpu_baseline(x, eps_text, m, n)
+
+    out = {
+        "metric": METRIC,
+        "value": n / (ms_per_step * 1e-3),
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded numpy generator, nextdbscan-exa_amd/synth.py)",
+        "config": {"workload": args.workload, "n": n, "d": d, "eps": float(eps), "min_pts": m, "path": path,
+                   "cells": stats["cells"], "tiles": stats["tiles"], "result": list(meta)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,589 @@
+// Cell grid for d <= 3: bounds, plan, cell keys, radix sort, cell table, row
+// hash, neighbour-cell stencil, counting tiles and the neighbour-count kernel.
+//
+// The grid has no counterpart in the reference, whose counting is all-pairs
+// (data_process.cpp:433-454).  It is a filter only: every candidate pair
+// still goes through the reference's float32 predicate, so nn is the same
+// integer.  Cell side s = sqrt(e2/d)*(1-2^-18): two points of one cell are
+// always within eps under that predicate (used by the label pass, never by
+// counting), and a stencil of +-reach cells per dimension contains every
+// point the predicate can accept.
+#include <cfloat>
+#include <cmath>
+
+#include "ndbx_internal.h"
+
+namespace ndbx {
+
+constexpr int64_t kBoundsChunk = 1 << 16;
+
+// ------------------------------------------------------------ bounds --
+__global__ __launch_bounds__(256) void k_bounds(const float *__restrict__ x, int d, int dg,
+                                                const int64_t *__restrict__ chunk_begin,
+                                                const int64_t *__restrict__ chunk_end,
+                                                BoundsPart *__restrict__ out) {
+    __shared__ float sh[6][256 / 64];
+    __shared__ int32_t shs[256 / 64];
+    const int64_t b = chunk_begin[blockIdx.x], e = chunk_end[blockIdx.x];
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    int32_t special = 0;
+    for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+        const float *p = x + i * d;
+        bool sp = p[0] == FLT_MAX;
+        for (int k = 0; k < d; ++k) sp |= !isfinite(p[k]);
+        if (sp) {
+            ++special;
+            continue;
+        }
+        for (int k = 0; k < dg; ++k) {
+            lo[k] = fminf(lo[k], p[k]);
+            hi[k] = fmaxf(hi[k], p[k]);
+        }
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        for (int off = 32; off >= 1; off >>= 1) {
+            lo[k] = fminf(lo[k], __shfl_xor(lo[k], off, 64));
+            hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], off, 64));
+        }
+    }
+    for (int off = 32; off >= 1; off >>= 1) special += __shfl_xor(special, off, 64);
+    if (lane == 0) {
+        for (int k = 0; k < 3; ++k) {
+            sh[k][w] = lo[k];
+            sh[3 + k][w] = hi[k];
+        }
+        shs[w] = special;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        BoundsPart r;
+        for (int k = 0; k < 3; ++k) {
+            r.lo[k] = sh[k][0];
+            r.hi[k] = sh[3 + k][0];
+        }
+        r.special = shs[0];
+        for (int q = 1; q < 4; ++q) {
+            for (int k = 0; k < 3; ++k) {
+                r.lo[k] = fminf(r.lo[k], sh[k][q]);
+                r.hi[k] = fmaxf(r.hi[k], sh[3 + k][q]);
+            }
+            r.special += shs[q];
+        }
+        r.count = (int32_t)(e - b);
+        out[blockIdx.x] = r;
+    }
+}
+
+static int bits_for(uint64_t v) {  // bits needed to hold values 0..v-1
+    int b = 0;
+    while (b < 64 && (v > (1ull << b))) ++b;
+    return b;
+}
+
+// Decides the path and, for the grid, each instance's cell geometry.
+void grid_plan(Ctx &c) {
+    c.path = PATH_GRID;
+    if (c.flags & NDBX_FORCE_BRUTE) c.path = PATH_BRUTE;
+    if (c.d > 3) c.path = PATH_BRUTE;
+    if (c.n == 0) {
+        c.h_inst.clear();
+        return;
+    }
+    const int dg = c.d < 3 ? c.d : 3;
+    // chunk table (each chunk inside one instance)
+    std::vector<int64_t> cb, ce;
+    std::vector<int> ci;
+    for (int i = 0; i < c.n_inst; ++i) {
+        for (int64_t b = c.h_offsets[i]; b < c.h_offsets[i + 1]; b += kBoundsChunk) {
+            cb.push_back(b);
+            ce.push_back(std::min(b + kBoundsChunk, c.h_offsets[i + 1]));
+            ci.push_back(i);
+        }
+    }
+    const int64_t nch = (int64_t)cb.size();
+    c.misc.ensure((size_t)nch * 2 * sizeof(int64_t));
+    c.bounds_part.ensure((size_t)nch * sizeof(BoundsPart));
+    int64_t *d_cb = c.misc.as<int64_t>();
+    NDBX_HIP(hipMemcpyAsync(d_cb, cb.data(), nch * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
+    NDBX_HIP(hipMemcpyAsync(d_cb + nch, ce.data(), nch * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
+    {
+        Launch L(c.prof, c.stream, "bounds");
+        k_bounds<<<(unsigned)nch, 256, 0, c.stream>>>(c.coords, c.d, dg, d_cb, d_cb + nch,
+                                                      c.bounds_part.as<BoundsPart>());
+    }
+    std::vector<BoundsPart> parts(nch);
+    NDBX_HIP(hipMemcpyAsync(parts.data(), c.bounds_part.p, nch * sizeof(BoundsPart), hipMemcpyDeviceToHost,
+                            c.stream));
+    c.sync();
+
+    c.h_inst.assign(c.n_inst, InstGrid{});
+    std::vector<float> lo(c.n_inst * 3, FLT_MAX), hi(c.n_inst * 3, -FLT_MAX);
+    std::vector<int64_t> special(c.n_inst, 0);
+    for (int64_t q = 0; q < nch; ++q) {
+        int i = ci[q];
+        special[i] += parts[q].special;
+        for (int k = 0; k < dg; ++k) {
+            lo[i * 3 + k] = std::min(lo[i * 3 + k], parts[q].lo[k]);
+            hi[i * 3 + k] = std::max(hi[i * 3 + k], parts[q].hi[k]);
+        }
+    }
+    int lin_bits = 1;
+    for (int i = 0; i < c.n_inst; ++i) {
+        InstGrid &g = c.h_inst[i];
+        const float e = c.h_eps[i];
+        const float e2 = e * e;  // data_process.h:73
+        g.e2 = e2;
+        g.m = c.h_m[i];
+        g.point_begin = c.h_offsets[i];
+        g.point_end = c.h_offsets[i + 1];
+        g.core_base = 0;
+        for (int k = 0; k < 3; ++k) {
+            g.origin[k] = 0.0;
+            g.G[k] = 1;
+        }
+        g.inv_s = 1.0;
+        g.reach = 2;
+        if (g.point_end == g.point_begin) continue;
+        if (special[i] > 0 || !std::isfinite(e2) || !(e2 >= 1e-30f)) {
+            c.path = PATH_BRUTE;
+            continue;
+        }
+        const double s = std::sqrt((double)e2 / (double)c.d) * (1.0 - std::ldexp(1.0, -18));
+        g.inv_s = 1.0 / s;
+        const double rmax = std::sqrt((double)e2 * (1.0 + std::ldexp(1.0, -20))) * (1.0 + std::ldexp(1.0, -30));
+        g.reach = (int32_t)std::ceil(rmax * g.inv_s);
+        uint64_t total = 1;
+        bool too_big = g.reach > 2;
+        for (int k = 0; k < dg; ++k) {
+            g.origin[k] = (double)lo[i * 3 + k];
+            const double span = ((double)hi[i * 3 + k] - g.origin[k]) * g.inv_s;
+            if (!(span < 2097152.0)) {  // 2^21 cells per dimension
+                too_big = true;
+                break;
+            }
+            g.G[k] = (int64_t)std::floor(span) + 1;
+            total *= (uint64_t)g.G[k];
+        }
+        if (too_big || total >= (1ull << 52)) {
+            c.path = PATH_BRUTE;
+            continue;
+        }
+        lin_bits = std::max(lin_bits, bits_for(total));
+    }
+    const int inst_bits = bits_for((uint64_t)c.n_inst);
+    c.lin_bits = lin_bits;
+    c.key_bits = lin_bits + inst_bits;
+    c.reach_max = 2;
+    if (c.key_bits > 62) c.path = PATH_BRUTE;
+}
+
+// -------------------------------------------------------------- keys --
+__device__ __forceinline__ int find_inst(const int64_t *__restrict__ off, int n_inst, int64_t i) {
+    int lo = 0, hi = n_inst - 1;
+    while (lo < hi) {  // last instance with off[inst] <= i
+        int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_keys(const float *__restrict__ x, int64_t n,
+                                              const InstGrid *__restrict__ ig,
+                                              const int64_t *__restrict__ off, int n_inst, int lin_bits,
+                                              uint64_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int inst = n_inst > 1 ? find_inst(off, n_inst, i) : 0;
+    const InstGrid &g = ig[inst];
+    uint64_t lin = 0;
+#pragma unroll
+    for (int k = D - 1; k >= 0; --k) {
+        double t = ((double)x[i * D + k] - g.origin[k]) * g.inv_s;
+        int64_t ck = (int64_t)floor(t);
+        ck = ck < 0 ? 0 : (ck >= g.G[k] ? g.G[k] - 1 : ck);
+        lin = lin * (uint64_t)g.G[k] + (uint64_t)ck;
+    }
+    keys[i] = ((uint64_t)inst << lin_bits) | lin;
+    vals[i] = (uint32_t)i;
+}
+
+// --------------------------------------------------------- cell table --
+__global__ __launch_bounds__(256) void k_cell_flags(const uint64_t *__restrict__ keys, int64_t n,
+                                                    int32_t *__restrict__ flag) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_cell_fill(const uint64_t *__restrict__ keys,
+                                                   const int32_t *__restrict__ flag,
+                                                   const int32_t *__restrict__ excl, int64_t n,
+                                                   int32_t *__restrict__ cell_of, int32_t *__restrict__ cell_start,
+                                                   uint64_t *__restrict__ cell_key) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t f = flag[i];
+    const int32_t cid = excl[i] + f - 1;
+    cell_of[i] = cid;
+    if (f) {
+        cell_start[cid] = (int32_t)i;
+        cell_key[cid] = keys[i];
+    }
+    if (i == n - 1) cell_start[cid + 1] = (int32_t)n;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_gather_soa(const float *__restrict__ x, const uint32_t *__restrict__ perm,
+                                                    int64_t n, int64_t npad, float *__restrict__ soa) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t p = perm[i];
+#pragma unroll
+    for (int k = 0; k < D; ++k) soa[k * npad + i] = x[p * D + k];
+}
+
+// ------------------------------------------------------------ row hash --
+__device__ __forceinline__ uint64_t mix64(uint64_t v) {
+    v ^= v >> 33;
+    v *= 0xff51afd7ed558ccdull;
+    v ^= v >> 33;
+    v *= 0xc4ceb9fe1a85ec53ull;
+    v ^= v >> 33;
+    return v;
+}
+
+constexpr uint64_t kEmpty = ~0ull;
+
+__device__ __forceinline__ uint64_t row_key_of(uint64_t key, int lin_bits, const InstGrid *__restrict__ ig) {
+    const uint64_t inst = key >> lin_bits;
+    const uint64_t lin = key & ((1ull << lin_bits) - 1);
+    return key - lin % (uint64_t)ig[inst].G[0];
+}
+
+__global__ __launch_bounds__(256) void k_row_insert(const uint64_t *__restrict__ cell_key, int64_t nc,
+                                                    const InstGrid *__restrict__ ig, int lin_bits,
+                                                    uint64_t *__restrict__ tkeys, int2 *__restrict__ tvals,
+                                                    uint64_t mask) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const uint64_t rk = row_key_of(cell_key[c], lin_bits, ig);
+    const bool first = (c == 0) || row_key_of(cell_key[c - 1], lin_bits, ig) != rk;
+    const bool last = (c == nc - 1) || row_key_of(cell_key[c + 1], lin_bits, ig) != rk;
+    if (!first && !last) return;
+    uint64_t slot = mix64(rk) & mask;
+    for (;;) {
+        unsigned long long prev = atomicCAS((unsigned long long *)&tkeys[slot], (unsigned long long)kEmpty,
+                                            (unsigned long long)rk);
+        if (prev == kEmpty || prev == rk) break;
+        slot = (slot + 1) & mask;
+    }
+    if (first) tvals[slot].x = (int32_t)c;
+    if (last) tvals[slot].y = (int32_t)(c + 1);
+}
+
+__device__ __forceinline__ bool row_lookup(const uint64_t *__restrict__ tkeys, const int2 *__restrict__ tvals,
+                                           uint64_t mask, uint64_t rk, int2 *out) {
+    uint64_t slot = mix64(rk) & mask;
+    for (;;) {
+        const uint64_t k = tkeys[slot];
+        if (k == rk) {
+            *out = tvals[slot];
+            return true;
+        }
+        if (k == kEmpty) return false;
+        slot = (slot + 1) & mask;
+    }
+}
+
+// ------------------------------------------------------------- stencil --
+// For every cell, the point ranges of the neighbour cells, one range per
+// stencil row (cells contiguous along dimension 0 are contiguous in sorted
+// order).
+template <int D>
+__global__ __launch_bounds__(256) void k_stencil(const uint64_t *__restrict__ cell_key,
+                                                 const int32_t *__restrict__ cell_start, int64_t nc,
+                                                 const InstGrid *__restrict__ ig, int lin_bits,
+                                                 const uint64_t *__restrict__ tkeys, const int2 *__restrict__ tvals,
+                                                 uint64_t mask, int32_t *__restrict__ nranges,
+                                                 int2 *__restrict__ ranges, int rows_per_cell,
+                                                 int32_t *__restrict__ work) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const uint64_t key = cell_key[c];
+    const uint64_t inst = key >> lin_bits;
+    const InstGrid &g = ig[inst];
+    uint64_t lin = key & ((1ull << lin_bits) - 1);
+    const int64_t G0 = g.G[0], G1 = g.G[1], G2 = g.G[2];
+    const int64_t c0 = (int64_t)(lin % (uint64_t)G0);
+    lin /= (uint64_t)G0;
+    const int64_t c1 = D >= 2 ? (int64_t)(lin % (uint64_t)G1) : 0;
+    const int64_t c2 = D >= 3 ? (int64_t)(lin / (uint64_t)G1) : 0;
+    const int k = g.reach;
+    const int64_t lo0 = c0 - k < 0 ? 0 : c0 - k;
+    const int64_t hi0 = c0 + k >= G0 ? G0 - 1 : c0 + k;
+    const uint64_t ibase = inst << lin_bits;
+    int nr = 0;
+    int64_t total = 0;
+    const int kz = D >= 3 ? k : 0, ky = D >= 2 ? k : 0;
+    for (int dz = -kz; dz <= kz; ++dz) {
+        const int64_t z = c2 + dz;
+        if (z < 0 || z >= G2) continue;
+        for (int dy = -ky; dy <= ky; ++dy) {
+            const int64_t y = c1 + dy;
+            if (y < 0 || y >= G1) continue;
+            const uint64_t rk = ibase | (uint64_t)((z * G1 + y) * G0);
+            int2 rv;
+            if (!row_lookup(tkeys, tvals, mask, rk, &rv)) continue;
+            // first cell with key >= rk+lo0
+            int a = rv.x, b = rv.y;
+            const uint64_t klo = rk + (uint64_t)lo0, khi = rk + (uint64_t)hi0;
+            while (a < b) {
+                int mid = (a + b) >> 1;
+                if (cell_key[mid] < klo) a = mid + 1;
+                else b = mid;
+            }
+            int e = a, f = rv.y;
+            while (e < f) {  // first cell with key > khi
+                int mid = (e + f) >> 1;
+                if (cell_key[mid] <= khi) e = mid + 1;
+                else f = mid;
+            }
+            if (a < e) {
+                const int2 r = make_int2(cell_start[a], cell_start[e]);
+                ranges[c * rows_per_cell + nr] = r;
+                total += r.y - r.x;
+                ++nr;
+            }
+        }
+    }
+    nranges[c] = nr;
+    const int32_t cnt = cell_start[c + 1] - cell_start[c];
+    // tiles of this cell (ceil(cnt / kTile)); the candidate total is kept for
+    // pair-test statistics
+    work[c] = (cnt + kTile - 1) / kTile;
+    (void)total;
+}
+
+__global__ __launch_bounds__(256) void k_tiles(const int32_t *__restrict__ cell_start,
+                                               const int32_t *__restrict__ tile_off, int64_t nc,
+                                               int2 *__restrict__ tiles) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int32_t b = cell_start[c], e = cell_start[c + 1];
+    int32_t t = tile_off[c];
+    for (int32_t q = b; q < e; q += kTile) tiles[t++] = make_int2((int32_t)c, q);
+}
+
+// -------------------------------------------------------------- count --
+// One wave per tile of <= 64 points of one cell.  The candidate loop is
+// wave-uniform, so candidate coordinates come through scalar loads and each
+// lane tests its own query point with the reference's arithmetic
+// (data_process.cpp:443-447): ascending d, no contraction, '<='.
+// Unrolled candidate block: U candidates [j, j+U) of a wave-uniform range.
+template <int D, int U>
+__device__ __forceinline__ int32_t count_block(const float (&qx)[D], const float *__restrict__ soa, int64_t npad,
+                                               int j, float e2) {
+    float c[D][U];
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+#pragma unroll
+        for (int u = 0; u < U; ++u) c[k][u] = soa[k * npad + j + u];
+    int32_t cnt = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        float len = 0.0f;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const float df = qx[k] - c[k][u];
+            len = __fadd_rn(len, __fmul_rn(df, df));
+        }
+        cnt += (len <= e2) ? 1 : 0;
+    }
+    return cnt;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_count(const int2 *__restrict__ tiles, int64_t n_tiles,
+                                               const int32_t *__restrict__ cell_start,
+                                               const uint64_t *__restrict__ cell_key,
+                                               const InstGrid *__restrict__ ig, int lin_bits,
+                                               const int32_t *__restrict__ nranges, const int2 *__restrict__ ranges,
+                                               int rows_per_cell, const float *__restrict__ soa, int64_t npad,
+                                               int32_t *__restrict__ nn_s) {
+    // wave index made provably uniform, so everything derived from it (tile,
+    // ranges, candidate addresses) lives in SGPRs and loads through the
+    // scalar cache
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+    if (t >= n_tiles) return;
+    const int lane = threadIdx.x & 63;
+    const int2 tile = tiles[t];
+    const int cell = tile.x;
+    const int q = tile.y + lane;
+    const bool valid = q < cell_start[cell + 1];
+    const float e2 = ig[cell_key[cell] >> lin_bits].e2;
+    float qx[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) qx[k] = soa[k * npad + (valid ? q : tile.y)];
+    int32_t cnt = 0;
+    const int nr = nranges[cell];
+    const int2 *rr = ranges + (int64_t)cell * rows_per_cell;
+    for (int r = 0; r < nr; ++r) {
+        const int2 rg = rr[r];
+        int j = rg.x;
+        for (; j + 8 <= rg.y; j += 8) cnt += count_block<D, 8>(qx, soa, npad, j, e2);
+        for (; j < rg.y; ++j) cnt += count_block<D, 1>(qx, soa, npad, j, e2);
+    }
+    if (valid) nn_s[q] = cnt;
+}
+
+// nn and core flags in input order (v_coord_nn, data_process.h:57), and core
+// flags in sorted order for the label pass.
+__global__ __launch_bounds__(256) void k_classify_out(const int32_t *__restrict__ nn_s,
+                                                      const uint32_t *__restrict__ perm,
+                                                      const int32_t *__restrict__ cell_of,
+                                                      const uint64_t *__restrict__ cell_key,
+                                                      const InstGrid *__restrict__ ig, int lin_bits, int64_t n,
+                                                      int32_t *__restrict__ nn, uint8_t *__restrict__ core,
+                                                      uint8_t *__restrict__ core_s) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t v = nn_s[i];
+    const bool c = v >= ig[cell_key[cell_of[i]] >> lin_bits].m;  // data_process.cpp:282
+    const uint32_t p = perm[i];
+    nn[p] = v;
+    core[p] = c;
+    core_s[i] = c;
+}
+
+// ------------------------------------------------------------ pipeline --
+static void upload_inst(Ctx &c) {
+    c.inst_d.ensure(sizeof(InstGrid) * c.n_inst);
+    NDBX_HIP(hipMemcpyAsync(c.inst_d.p, c.h_inst.data(), sizeof(InstGrid) * c.n_inst, hipMemcpyHostToDevice,
+                            c.stream));
+    c.offsets_d.ensure(sizeof(int64_t) * (c.n_inst + 1));
+    NDBX_HIP(hipMemcpyAsync(c.offsets_d.p, c.h_offsets.data(), sizeof(int64_t) * (c.n_inst + 1),
+                            hipMemcpyHostToDevice, c.stream));
+}
+
+template <int D>
+static void grid_classify_d(Ctx &c) {
+    const int64_t n = c.n;
+    const hipStream_t s = c.stream;
+    const int nb = (int)ceil_div(n, 256);
+    upload_inst(c);
+    const InstGrid *ig = c.inst_d.as<InstGrid>();
+
+    c.keys.ensure(n * 8);
+    c.vals.ensure(n * 4);
+    c.keys2.ensure(n * 8);
+    c.vals2.ensure(n * 4);
+    {
+        Launch L(c.prof, s, "keys");
+        k_keys<D><<<nb, 256, 0, s>>>(c.coords, n, ig, c.offsets_d.as<int64_t>(), c.n_inst, c.lin_bits,
+                                    c.keys.as<uint64_t>(), c.vals.as<uint32_t>());
+    }
+    bool alt = false;
+    radix_sort_u64(c.keys.as<uint64_t>(), c.vals.as<uint32_t>(), c.keys2.as<uint64_t>(), c.vals2.as<uint32_t>(), n,
+                   c.key_bits, c.scratch, s, c.prof, &alt);
+    if (alt) {
+        std::swap(c.keys, c.keys2);
+        std::swap(c.vals, c.vals2);
+    }
+    const uint64_t *keys = c.keys.as<uint64_t>();
+    const uint32_t *perm = c.vals.as<uint32_t>();
+
+    c.flag.ensure(n * 4);
+    c.cid.ensure(n * 4);
+    c.cell_of.ensure(n * 4);
+    c.cell_start.ensure((n + 1) * 4);
+    c.cell_key.ensure(n * 8);
+    c.misc.ensure(64);
+    int32_t *d_total = c.misc.as<int32_t>();
+    {
+        Launch L(c.prof, s, "cell_flags");
+        k_cell_flags<<<nb, 256, 0, s>>>(keys, n, c.flag.as<int32_t>());
+    }
+    scan_exclusive_i32(c.flag.as<int32_t>(), c.cid.as<int32_t>(), n, d_total, c.scratch, s, c.prof);
+    {
+        Launch L(c.prof, s, "cell_fill");
+        k_cell_fill<<<nb, 256, 0, s>>>(keys, c.flag.as<int32_t>(), c.cid.as<int32_t>(), n, c.cell_of.as<int32_t>(),
+                                       c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>());
+    }
+    const int64_t npad = ceil_div(n, 64) * 64;
+    c.soa.ensure((size_t)npad * D * 4);
+    {
+        Launch L(c.prof, s, "gather_soa");
+        k_gather_soa<D><<<nb, 256, 0, s>>>(c.coords, perm, n, npad, c.soa.as<float>());
+    }
+    int32_t h_total = 0;
+    NDBX_HIP(hipMemcpyAsync(&h_total, d_total, 4, hipMemcpyDeviceToHost, s));
+    c.sync();
+    const int64_t nc = h_total;
+    c.n_cells = nc;
+    const int ncb = (int)ceil_div(nc, 256);
+
+    // row hash: power of two >= 2 * cells
+    uint64_t cap = 1024;
+    while (cap < (uint64_t)(2 * nc)) cap <<= 1;
+    c.row_keys.ensure(cap * 8);
+    c.row_vals.ensure(cap * 8);
+    fill_u64(c.row_keys.as<uint64_t>(), kEmpty, (int64_t)cap, s, c.prof);
+    {
+        Launch L(c.prof, s, "row_insert");
+        k_row_insert<<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), nc, ig, c.lin_bits, c.row_keys.as<uint64_t>(),
+                                         c.row_vals.as<int2>(), cap - 1);
+    }
+    int rows = 1;
+    for (int k = 1; k < D; ++k) rows *= 2 * c.reach_max + 1;
+    c.nranges.ensure(nc * 4);
+    c.ranges.ensure((size_t)nc * rows * 8);
+    c.tile_off.ensure((nc + 1) * 4);
+    c.cid.ensure(nc * 4);
+    int32_t *work = c.cid.as<int32_t>();  // reused: tiles per cell
+    {
+        Launch L(c.prof, s, "stencil");
+        k_stencil<D><<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), c.cell_start.as<int32_t>(), nc, ig, c.lin_bits,
+                                         c.row_keys.as<uint64_t>(), c.row_vals.as<int2>(), cap - 1,
+                                         c.nranges.as<int32_t>(), c.ranges.as<int2>(), rows, work);
+    }
+    scan_exclusive_i32(work, c.tile_off.as<int32_t>(), nc, d_total, c.scratch, s, c.prof);
+    NDBX_HIP(hipMemcpyAsync(&h_total, d_total, 4, hipMemcpyDeviceToHost, s));
+    c.sync();
+    const int64_t nt = h_total;
+    c.n_tiles = nt;
+    c.tiles.ensure(nt * 8);
+    {
+        Launch L(c.prof, s, "tiles");
+        k_tiles<<<ncb, 256, 0, s>>>(c.cell_start.as<int32_t>(), c.tile_off.as<int32_t>(), nc, c.tiles.as<int2>());
+    }
+    c.nn_s.ensure(n * 4);
+    {
+        Launch L(c.prof, s, "count");
+        k_count<D><<<(unsigned)ceil_div(nt, 4), 256, 0, s>>>(
+            c.tiles.as<int2>(), nt, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig, c.lin_bits,
+            c.nranges.as<int32_t>(), c.ranges.as<int2>(), rows, c.soa.as<float>(), npad, c.nn_s.as<int32_t>());
+    }
+    c.core_s.ensure(n);
+    {
+        Launch L(c.prof, s, "classify_out");
+        k_classify_out<<<nb, 256, 0, s>>>(c.nn_s.as<int32_t>(), perm, c.cell_of.as<int32_t>(), c.cell_key.as<uint64_t>(),
+                                          ig, c.lin_bits, n, c.nn.as<int32_t>(), c.core.as<uint8_t>(),
+                                          c.core_s.as<uint8_t>());
+    }
+}
+
+void grid_classify(Ctx &c) {
+    switch (c.d) {
+        case 1: grid_classify_d<1>(c); break;
+        case 2: grid_classify_d<2>(c); break;
+        case 3: grid_classify_d<3>(c); break;
+        default: throw Error(NDBX_EUNSUPPORTED, "grid path needs d <= 3");
+    }
+}
+
+}  // namespace ndbx

@@ -1,0 +1,169 @@
+// Internal declarations of the ndbx library (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "../../include/ndbx.h"
+
+namespace ndbx {
+
+// ---------------------------------------------------------------- errors --
+struct Error : std::exception {
+    int code;
+    std::string msg;
+    Error(int c, std::string m) : code(c), msg(std::move(m)) {}
+    const char *what() const noexcept override { return msg.c_str(); }
+};
+
+#define NDBX_HIP(call)                                                          \
+    do {                                                                        \
+        hipError_t e_ = (call);                                                 \
+        if (e_ != hipSuccess)                                                   \
+            throw ::ndbx::Error(e_ == hipErrorOutOfMemory ? NDBX_ENOMEM         \
+                                                          : NDBX_EDEVICE,       \
+                                std::string(#call) + ": " +                     \
+                                    hipGetErrorString(e_));                     \
+    } while (0)
+
+// ------------------------------------------------------- device buffers --
+// Grow-only device allocation owned by a context.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    template <class T>
+    T *as() const { return static_cast<T *>(p); }
+    void *ensure(size_t want);
+    void release();
+};
+
+// ------------------------------------------------------------- profiler --
+struct Profiler {
+    bool on = false;
+    struct Pending {
+        int slot;
+        hipEvent_t a, b;
+    };
+    std::vector<std::string> names;
+    std::vector<double> ms;
+    std::vector<int64_t> calls;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    int slot_of(const char *name);
+    hipEvent_t take();
+    void resolve();  // after the stream is idle
+    void reset();
+    void destroy();
+};
+
+// Brackets one kernel launch with events when profiling is on.
+struct Launch {
+    Profiler &prof;
+    hipStream_t s;
+    int slot = -1;
+    hipEvent_t a{}, b{};
+    Launch(Profiler &p, hipStream_t st, const char *name);
+    ~Launch() noexcept(false);
+};
+
+// --------------------------------------------------------- grid params --
+// One DBSCAN instance's grid: cell side s, origin at the instance's minimum,
+// G[k] cells along each grid dimension.  Cell keys are
+//   key = (instance << lin_bits) | ((c2*G1 + c1)*G0 + c0).
+struct InstGrid {
+    double origin[3];
+    double inv_s;
+    int64_t G[3];
+    int64_t point_begin;  // first point of the instance (input order)
+    int64_t point_end;
+    float e2;
+    int32_t m;
+    int32_t reach;  // stencil half-width in cells
+    int32_t core_base;  // cores before this instance (input order), set on device
+};
+
+// Per-chunk bounds partials: min[3], max[3], special count.
+struct BoundsPart {
+    float lo[3];
+    float hi[3];
+    int32_t special;
+    int32_t count;
+};
+
+constexpr int kTile = 64;         // points per counting tile (one wave)
+constexpr int kMaxRowsPerCell = 25;  // (2*2+1)^(3-1) stencil rows for d<=3
+
+// ----------------------------------------------------------- primitives --
+// Exclusive scan of int32; `total` (device, may be null) receives the sum.
+void scan_exclusive_i32(const int32_t *in, int32_t *out, int64_t n, int32_t *total,
+                        DevBuf &scratch, hipStream_t s, Profiler &prof);
+// Stable LSD radix sort of (key, val) on key bits [0, nbits).
+void radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
+                    int64_t n, int nbits, DevBuf &scratch, hipStream_t s, Profiler &prof,
+                    bool *result_in_alt);
+void fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s, Profiler &prof);
+void fill_u64(uint64_t *p, uint64_t v, int64_t n, hipStream_t s, Profiler &prof);
+
+// ------------------------------------------------------------- context --
+enum Path { PATH_GRID = 0, PATH_BRUTE = 1, PATH_MFMA = 2 };
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t n = 0;
+    int d = 0;
+    uint32_t flags = 0;
+    int n_inst = 1;
+    std::vector<int64_t> h_offsets;  // n_inst+1
+    std::vector<float> h_eps;
+    std::vector<int32_t> h_m;
+    bool classified = false, labelled = false;
+    int path = PATH_GRID;
+
+    const float *coords = nullptr;  // device AoS n*d
+    DevBuf own_coords;
+
+    // results, input order
+    DevBuf nn, labels, core;  // int32, int32, uint8
+    DevBuf meta;              // int32[3*n_inst]
+    std::vector<int32_t> h_meta;
+    bool meta_valid = false;
+
+    // grid path state
+    DevBuf offsets_d, inst_d, bounds_part, keys, vals, keys2, vals2, flag, cid, cell_of,
+        cell_start, cell_key, soa, row_keys, row_vals, nranges, ranges, tile_off, tiles, nn_s,
+        core_s, cell_core_min, cell_has_core, parent, comp_min, cell_label, label_s, rank,
+        misc, scratch, scratch2;
+    int64_t n_cells = 0, n_tiles = 0, pair_tests = 0;
+    int lin_bits = 0, key_bits = 0, reach_max = 2;
+    std::vector<InstGrid> h_inst;
+
+    // brute / mfma path state
+    DevBuf b_parent, b_rootmin;
+
+    // audit (mfma)
+    int64_t audit_band = 0, audit_fp64_flips = 0, audit_filter_flips = 0;
+    double audit_max_err = 0.0;
+
+    Profiler prof;
+
+    void sync();
+    ~Ctx();
+};
+
+// Pipelines (each file owns its kernels).
+void grid_plan(Ctx &c);            // bounds -> path choice and per-instance grids
+void grid_classify(Ctx &c);        // keys, sort, cells, stencil, counts
+void grid_label(Ctx &c);           // union-find over cells, border, labels
+void brute_classify(Ctx &c);
+void brute_label(Ctx &c);
+void finish_meta(Ctx &c);          // meta reduction (both paths)
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace ndbx

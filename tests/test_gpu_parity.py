@@ -1,0 +1,127 @@
+"""Parity of the HIP path against the reference's outputs (golden fixtures) and
+the oracle, through the C ABI.  Integer results must be bit-exact: nn, core
+flags, noise set, and the cluster index of every point (cores: rank of the
+lowest-id core of the component; borders: smallest adjacent cluster index)."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+import synth
+from conftest import golden_names, load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def run(ndbx, x, eps, m, flags=0):
+    with ndbx.Context(x, eps, m, flags) as c:
+        c.classify()
+        nn_cls = c.nn()
+        c.label()
+        return nn_cls, c.nn(), c.labels(), c.meta(), c.core(), c.path()
+
+
+@pytest.mark.parametrize("force_brute", [False, True])
+@pytest.mark.parametrize("name", golden_names())
+def test_golden_fixture(gpu_lib, name, force_brute):
+    g = load_golden(name)
+    eps = synth.eps_f32(g["eps_text"])
+    flags = gpu_lib.FORCE_BRUTE if force_brute else 0
+    print(f"case {name} brute={force_brute} n={g['x'].shape}", flush=True)
+    nn0, nn, lab, meta, core, path = run(gpu_lib, g["x"], eps, g["m"], flags)
+    np.testing.assert_array_equal(nn0, g["nn"])
+    np.testing.assert_array_equal(nn, g["nn"])
+    np.testing.assert_array_equal(core, g["nn"] >= g["m"])
+    np.testing.assert_array_equal(lab, g["cci"])
+    assert meta == g["meta"]
+    if force_brute:
+        assert path == "all-pairs"
+
+
+RANDOM_CASES = [
+    # (d, n, n_blobs, extent, sigma range, noise, eps text, m, seed)
+    (2, 5000, 8, 50.0, (0.5, 2.0), 0.1, "0.5", 10, 1),
+    (2, 20000, 30, 100.0, (0.3, 1.0), 0.05, "0.3", 12, 2),
+    (3, 8000, 10, 60.0, (1.0, 5.0), 0.02, "1.5", 20, 3),
+    (3, 20000, 40, 100.0, (0.5, 3.0), 0.05, "1.0", 8, 4),
+    (1, 6000, 10, 100.0, (0.1, 1.0), 0.05, "0.05", 6, 5),
+    (3, 3000, 3, 10.0, (0.2, 0.5), 0.0, "0.4", 30, 6),   # dense cells (> 64 points per cell)
+    (2, 4000, 2, 5.0, (0.05, 0.1), 0.0, "0.2", 50, 7),   # very dense: hundreds per cell
+]
+
+
+@pytest.mark.parametrize("case", RANDOM_CASES, ids=lambda c: f"d{c[0]}_n{c[1]}_s{c[8]}")
+def test_random_against_oracle(gpu_lib, case):
+    d, n, nb, ext, sr, noise, eps_t, m, seed = case
+    x = synth.blobs(n, d, nb, 0.0, ext, None, noise, 500 + seed, sigma_range=sr)
+    eps = synth.eps_f32(eps_t)
+    nn_o, lab_o, meta_o = O.dbscan(x, eps, m)
+    print(f"case random d={d} n={n} seed={seed}", flush=True)
+    _, nn, lab, meta, _, path = run(gpu_lib, x, eps, m)
+    assert path == "grid"
+    np.testing.assert_array_equal(nn, nn_o)
+    np.testing.assert_array_equal(lab, lab_o)
+    assert meta == meta_o
+
+
+def test_blobs2d_100k_against_oracle(gpu_lib):
+    x, eps, m = synth.make("blobs2d_100k")
+    print("case blobs2d_100k", flush=True)
+    _, nn, lab, meta, _, path = run(gpu_lib, x, eps, m)
+    nn_o, lab_o, meta_o = O.dbscan(x, eps, m)
+    assert path == "grid"
+    np.testing.assert_array_equal(nn, nn_o)
+    np.testing.assert_array_equal(lab, lab_o)
+    assert meta == meta_o
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 63, 64, 65])
+def test_tiny_inputs(gpu_lib, n):
+    rng = np.random.default_rng(n)
+    x = rng.uniform(0, 1, (n, 2)).astype(np.float32)
+    eps = np.float32(0.3)
+    print(f"case tiny n={n}", flush=True)
+    _, nn, lab, meta, _, _ = run(gpu_lib, x, eps, 2)
+    nn_o, lab_o, meta_o = O.dbscan(x, eps, 2)
+    np.testing.assert_array_equal(nn, nn_o)
+    np.testing.assert_array_equal(lab, lab_o)
+    assert meta == meta_o
+
+
+def test_degenerate_extent_and_duplicates(gpu_lib):
+    # every point on one line (zero extent in two dims), many duplicates
+    rng = np.random.default_rng(9)
+    t = np.round(rng.uniform(0, 20, 3000), 1).astype(np.float32)
+    x = np.stack([t, np.zeros_like(t), np.full_like(t, 7.0)], 1)
+    eps = np.float32(0.1)
+    print("case line-duplicates", flush=True)
+    _, nn, lab, meta, _, path = run(gpu_lib, x, eps, 5)
+    nn_o, lab_o, meta_o = O.dbscan(x, eps, 5)
+    assert path == "grid"
+    np.testing.assert_array_equal(nn, nn_o)
+    np.testing.assert_array_equal(lab, lab_o)
+    assert meta == meta_o
+
+
+def test_rerun_is_identical(gpu_lib):
+    x, eps, m = synth.make("blobs2d_100k", n=30000)
+    with gpu_lib.Context(x, eps, m) as c:
+        c.classify().label()
+        a = (c.nn(), c.labels(), c.meta())
+        c.classify().label()
+        b = (c.nn(), c.labels(), c.meta())
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    assert a[2] == b[2]
+
+
+def test_state_errors(gpu_lib):
+    x = np.zeros((10, 2), np.float32)
+    c = gpu_lib.Context(x, 0.5, 2)
+    with pytest.raises(gpu_lib.NdbxError) as e:
+        c.label()
+    assert e.value.code == gpu_lib.ESTATE
+    with pytest.raises(gpu_lib.NdbxError):
+        c.labels()
+    c.classify().label()
+    assert c.meta() == (10, 0, 1)
+    c.close()
