@@ -170,7 +170,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded numpy generator, nextdbscan-exa_amd/synth.py)",
         "config": {"workload": args.workload, "n": n, "d": d, "eps": float(eps), "min_pts": m, "path": path,
-                   "cells": stats["cells"], "tiles": stats["tiles"], "result": list(meta)},
+                   "cells": stats["cells"], "tiles": stats["tiles"], "pair_tests": stats["pair_tests"], "result": list(meta)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,

@@ -174,9 +174,13 @@ void grid_plan(Ctx &c) {
     }
     const int inst_bits = bits_for((uint64_t)c.n_inst);
     c.lin_bits = lin_bits;
-    c.key_bits = lin_bits + inst_bits;
+    c.cell_bits = lin_bits + inst_bits;
+    // 2 bits per dimension of sub-cell position below the cell key, so that
+    // within a cell the points are in Morton order of a 4^d sub-grid
+    c.sub_bits = c.cell_bits + 2 * dg <= 62 ? 2 * dg : 0;
+    c.key_bits = c.cell_bits + c.sub_bits;
     c.reach_max = 2;
-    if (c.key_bits > 62) c.path = PATH_BRUTE;
+    if (c.cell_bits > 62) c.path = PATH_BRUTE;
 }
 
 // -------------------------------------------------------------- keys --
@@ -194,32 +198,39 @@ template <int D>
 __global__ __launch_bounds__(256) void k_keys(const float *__restrict__ x, int64_t n,
                                               const InstGrid *__restrict__ ig,
                                               const int64_t *__restrict__ off, int n_inst, int lin_bits,
-                                              uint64_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+                                              int sub_bits, uint64_t *__restrict__ keys,
+                                              uint32_t *__restrict__ vals) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int inst = n_inst > 1 ? find_inst(off, n_inst, i) : 0;
     const InstGrid &g = ig[inst];
     uint64_t lin = 0;
+    uint32_t morton = 0;
 #pragma unroll
     for (int k = D - 1; k >= 0; --k) {
-        double t = ((double)x[i * D + k] - g.origin[k]) * g.inv_s;
+        const double t = ((double)x[i * D + k] - g.origin[k]) * g.inv_s;
         int64_t ck = (int64_t)floor(t);
         ck = ck < 0 ? 0 : (ck >= g.G[k] ? g.G[k] - 1 : ck);
         lin = lin * (uint64_t)g.G[k] + (uint64_t)ck;
+        int sc = (int)floor((t - (double)ck) * 4.0);
+        sc = sc < 0 ? 0 : (sc > 3 ? 3 : sc);
+        morton |= (uint32_t)(sc & 1) << k;          // bit 0 of each dimension
+        morton |= (uint32_t)(sc >> 1) << (D + k);   // bit 1 of each dimension
     }
-    keys[i] = ((uint64_t)inst << lin_bits) | lin;
+    const uint64_t cell = ((uint64_t)inst << lin_bits) | lin;
+    keys[i] = sub_bits ? (cell << sub_bits) | morton : cell;
     vals[i] = (uint32_t)i;
 }
 
 // --------------------------------------------------------- cell table --
-__global__ __launch_bounds__(256) void k_cell_flags(const uint64_t *__restrict__ keys, int64_t n,
+__global__ __launch_bounds__(256) void k_cell_flags(const uint64_t *__restrict__ keys, int64_t n, int sub_bits,
                                                     int32_t *__restrict__ flag) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
+    flag[i] = (i == 0 || (keys[i] >> sub_bits) != (keys[i - 1] >> sub_bits)) ? 1 : 0;
 }
 
-__global__ __launch_bounds__(256) void k_cell_fill(const uint64_t *__restrict__ keys,
+__global__ __launch_bounds__(256) void k_cell_fill(const uint64_t *__restrict__ keys, int sub_bits,
                                                    const int32_t *__restrict__ flag,
                                                    const int32_t *__restrict__ excl, int64_t n,
                                                    int32_t *__restrict__ cell_of, int32_t *__restrict__ cell_start,
@@ -231,7 +242,7 @@ __global__ __launch_bounds__(256) void k_cell_fill(const uint64_t *__restrict__ 
     cell_of[i] = cid;
     if (f) {
         cell_start[cid] = (int32_t)i;
-        cell_key[cid] = keys[i];
+        cell_key[cid] = keys[i] >> sub_bits;
     }
     if (i == n - 1) cell_start[cid + 1] = (int32_t)n;
 }
@@ -300,17 +311,19 @@ __device__ __forceinline__ bool row_lookup(const uint64_t *__restrict__ tkeys, c
 }
 
 // ------------------------------------------------------------- stencil --
-// For every cell, the point ranges of the neighbour cells, one range per
-// stencil row (cells contiguous along dimension 0 are contiguous in sorted
-// order).
+// For every cell, one slot per stencil row (dz, dy) in [-reach, reach]^(D-1):
+// the neighbour cells of that row with c0' in [c0-reach, c0+reach], as a cell
+// index range [a, e).  Cells contiguous along dimension 0 are contiguous in
+// sorted order, and so are their points, [cell_start[a], cell_start[e]).  An
+// empty window keeps its insertion position (a == e), so the window of a run
+// of cells [cf, cl] of one row is [slot(cf).a, slot(cl).e).
 template <int D>
 __global__ __launch_bounds__(256) void k_stencil(const uint64_t *__restrict__ cell_key,
                                                  const int32_t *__restrict__ cell_start, int64_t nc,
                                                  const InstGrid *__restrict__ ig, int lin_bits,
                                                  const uint64_t *__restrict__ tkeys, const int2 *__restrict__ tvals,
-                                                 uint64_t mask, int32_t *__restrict__ nranges,
-                                                 int2 *__restrict__ ranges, int rows_per_cell,
-                                                 int32_t *__restrict__ work) {
+                                                 uint64_t mask, int2 *__restrict__ ranges, int rows_per_cell,
+                                                 int32_t *__restrict__ cell_c, int32_t *__restrict__ work) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const uint64_t key = cell_key[c];
@@ -322,50 +335,44 @@ __global__ __launch_bounds__(256) void k_stencil(const uint64_t *__restrict__ ce
     lin /= (uint64_t)G0;
     const int64_t c1 = D >= 2 ? (int64_t)(lin % (uint64_t)G1) : 0;
     const int64_t c2 = D >= 3 ? (int64_t)(lin / (uint64_t)G1) : 0;
+    cell_c[c] = (int32_t)c0;
+    cell_c[nc + c] = (int32_t)c1;
+    cell_c[2 * nc + c] = (int32_t)c2;
     const int k = g.reach;
     const int64_t lo0 = c0 - k < 0 ? 0 : c0 - k;
     const int64_t hi0 = c0 + k >= G0 ? G0 - 1 : c0 + k;
     const uint64_t ibase = inst << lin_bits;
-    int nr = 0;
-    int64_t total = 0;
     const int kz = D >= 3 ? k : 0, ky = D >= 2 ? k : 0;
+    int slot = 0;
     for (int dz = -kz; dz <= kz; ++dz) {
-        const int64_t z = c2 + dz;
-        if (z < 0 || z >= G2) continue;
-        for (int dy = -ky; dy <= ky; ++dy) {
-            const int64_t y = c1 + dy;
-            if (y < 0 || y >= G1) continue;
-            const uint64_t rk = ibase | (uint64_t)((z * G1 + y) * G0);
+        for (int dy = -ky; dy <= ky; ++dy, ++slot) {
+            int2 out = make_int2(0, 0);
+            const int64_t z = c2 + dz, y = c1 + dy;
             int2 rv;
-            if (!row_lookup(tkeys, tvals, mask, rk, &rv)) continue;
-            // first cell with key >= rk+lo0
-            int a = rv.x, b = rv.y;
-            const uint64_t klo = rk + (uint64_t)lo0, khi = rk + (uint64_t)hi0;
-            while (a < b) {
-                int mid = (a + b) >> 1;
-                if (cell_key[mid] < klo) a = mid + 1;
-                else b = mid;
+            if (z >= 0 && z < G2 && y >= 0 && y < G1) {
+                const uint64_t rk = ibase | (uint64_t)((z * G1 + y) * G0);
+                if (row_lookup(tkeys, tvals, mask, rk, &rv)) {
+                    int a = rv.x, b = rv.y;  // first cell with key >= rk+lo0
+                    const uint64_t klo = rk + (uint64_t)lo0, khi = rk + (uint64_t)hi0;
+                    while (a < b) {
+                        int mid = (a + b) >> 1;
+                        if (cell_key[mid] < klo) a = mid + 1;
+                        else b = mid;
+                    }
+                    int e = a, f = rv.y;
+                    while (e < f) {  // first cell with key > khi
+                        int mid = (e + f) >> 1;
+                        if (cell_key[mid] <= khi) e = mid + 1;
+                        else f = mid;
+                    }
+                    out = make_int2(a, e);
+                }
             }
-            int e = a, f = rv.y;
-            while (e < f) {  // first cell with key > khi
-                int mid = (e + f) >> 1;
-                if (cell_key[mid] <= khi) e = mid + 1;
-                else f = mid;
-            }
-            if (a < e) {
-                const int2 r = make_int2(cell_start[a], cell_start[e]);
-                ranges[c * rows_per_cell + nr] = r;
-                total += r.y - r.x;
-                ++nr;
-            }
+            ranges[c * rows_per_cell + slot] = out;
         }
     }
-    nranges[c] = nr;
     const int32_t cnt = cell_start[c + 1] - cell_start[c];
-    // tiles of this cell (ceil(cnt / kTile)); the candidate total is kept for
-    // pair-test statistics
-    work[c] = (cnt + kTile - 1) / kTile;
-    (void)total;
+    work[c] = (cnt + kTile - 1) / kTile;  // label-pass tiles of this cell
 }
 
 __global__ __launch_bounds__(256) void k_tiles(const int32_t *__restrict__ cell_start,
@@ -376,6 +383,57 @@ __global__ __launch_bounds__(256) void k_tiles(const int32_t *__restrict__ cell_
     const int32_t b = cell_start[c], e = cell_start[c + 1];
     int32_t t = tile_off[c];
     for (int32_t q = b; q < e; q += kTile) tiles[t++] = make_int2((int32_t)c, q);
+}
+
+// Counting groups: the cells of one row whose c0 share c0 / kGroupCells; the
+// group's points are contiguous in sorted order and are cut into tiles of at
+// most 64 points (one wave each).
+constexpr int kGroupCells = 2;
+
+__global__ __launch_bounds__(256) void k_group_flags(const uint64_t *__restrict__ cell_key,
+                                                     const int32_t *__restrict__ cell_c, int64_t nc,
+                                                     int32_t *__restrict__ gflag) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    bool f = c == 0;
+    if (!f) {
+        const uint64_t rk = cell_key[c] - (uint64_t)cell_c[c], rk_prev = cell_key[c - 1] - (uint64_t)cell_c[c - 1];
+        f = rk != rk_prev || cell_c[c] / kGroupCells != cell_c[c - 1] / kGroupCells;
+    }
+    gflag[c] = f;
+}
+
+// Per cell that opens a group: number of tiles of the group.
+__global__ __launch_bounds__(256) void k_group_tiles(const int32_t *__restrict__ gflag,
+                                                     const int32_t *__restrict__ cell_start, int64_t nc,
+                                                     int32_t *__restrict__ ntiles) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    int32_t t = 0;
+    if (gflag[c]) {
+        int64_t e = c + 1;
+        while (e < nc && !gflag[e]) ++e;
+        t = (cell_start[e] - cell_start[c] + kTile - 1) / kTile;
+    }
+    ntiles[c] = t;
+}
+
+// Count tiles: (first cell, last cell, first point, points).
+__global__ __launch_bounds__(256) void k_group_tile_fill(const int32_t *__restrict__ gflag,
+                                                         const int32_t *__restrict__ cell_start,
+                                                         const int32_t *__restrict__ cell_of,
+                                                         const int32_t *__restrict__ toff, int64_t nc,
+                                                         int4 *__restrict__ ctiles) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc || !gflag[c]) return;
+    int64_t e = c + 1;
+    while (e < nc && !gflag[e]) ++e;
+    const int32_t p1 = cell_start[e];
+    int32_t t = toff[c];
+    for (int32_t p = cell_start[c]; p < p1; p += kTile) {
+        const int32_t len = p1 - p < kTile ? p1 - p : kTile;
+        ctiles[t++] = make_int4(cell_of[p], cell_of[p + len - 1], p, len);
+    }
 }
 
 // -------------------------------------------------------------- count --
@@ -406,39 +464,89 @@ __device__ __forceinline__ int32_t count_block(const float (&qx)[D], const float
     return cnt;
 }
 
+__device__ __forceinline__ float wave_min(float v) {
+    for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// One wave per count tile (<= 64 points of a run of cells of one row).  For
+// every stencil row the tile's bounding box first narrows the window: a row
+// whose perpendicular gap exceeds the search radius is skipped, and along
+// dimension 0 only cells that can hold a point within the radius of the box
+// are kept.  The radius is sqrt(e2 (1 + 2^-18)), above any distance the
+// float32 predicate can accept, and cell bounds are widened by 2^-20 of a
+// cell, so no neighbour is dropped.  The candidate loop is wave-uniform
+// (scalar loads) and each lane tests its own point with the reference's
+// arithmetic (data_process.cpp:443-447): ascending d, no contraction, '<='.
 template <int D>
-__global__ __launch_bounds__(256) void k_count(const int2 *__restrict__ tiles, int64_t n_tiles,
+__global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, int64_t n_tiles,
                                                const int32_t *__restrict__ cell_start,
                                                const uint64_t *__restrict__ cell_key,
                                                const InstGrid *__restrict__ ig, int lin_bits,
-                                               const int32_t *__restrict__ nranges, const int2 *__restrict__ ranges,
-                                               int rows_per_cell, const float *__restrict__ soa, int64_t npad,
-                                               int32_t *__restrict__ nn_s) {
-    // wave index made provably uniform, so everything derived from it (tile,
-    // ranges, candidate addresses) lives in SGPRs and loads through the
-    // scalar cache
+                                               const int2 *__restrict__ ranges, int rows_per_cell,
+                                               const int32_t *__restrict__ cell_c, int64_t nc,
+                                               const float *__restrict__ soa, int64_t npad,
+                                               int32_t *__restrict__ nn_s,
+                                               unsigned long long *__restrict__ tests) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
     if (t >= n_tiles) return;
     const int lane = threadIdx.x & 63;
-    const int2 tile = tiles[t];
-    const int cell = tile.x;
-    const int q = tile.y + lane;
-    const bool valid = q < cell_start[cell + 1];
-    const float e2 = ig[cell_key[cell] >> lin_bits].e2;
-    float qx[D];
+    const int4 tile = ctiles[t];
+    const int cf = tile.x, cl = tile.y;
+    const bool valid = lane < tile.w;
+    const int q = tile.z + (valid ? lane : 0);
+    float qx[D], lo[D], hi[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k) qx[k] = soa[k * npad + (valid ? q : tile.y)];
+    for (int k = 0; k < D; ++k) {
+        qx[k] = soa[k * npad + q];
+        lo[k] = wave_min(qx[k]);
+        hi[k] = wave_max(qx[k]);
+    }
+    const InstGrid &g = ig[cell_key[cf] >> lin_bits];
+    const float e2 = g.e2;
+    const double inv_s = g.inv_s, s = 1.0 / inv_s, slack = s * 9.5367431640625e-07;  // 2^-20 cell
+    const double r2 = (double)e2 * (1.0 + 3.814697265625e-06);                        // (1 + 2^-18)
+    const int c1 = cell_c[nc + cf], c2 = cell_c[2 * nc + cf];
     int32_t cnt = 0;
-    const int nr = nranges[cell];
-    const int2 *rr = ranges + (int64_t)cell * rows_per_cell;
-    for (int r = 0; r < nr; ++r) {
-        const int2 rg = rr[r];
-        int j = rg.x;
-        for (; j + 8 <= rg.y; j += 8) cnt += count_block<D, 8>(qx, soa, npad, j, e2);
-        for (; j < rg.y; ++j) cnt += count_block<D, 1>(qx, soa, npad, j, e2);
+    int64_t issued = 0;
+    const int2 *rf = ranges + (int64_t)cf * rows_per_cell;
+    const int2 *rl = ranges + (int64_t)cl * rows_per_cell;
+    for (int slot = 0; slot < rows_per_cell; ++slot) {
+        int a = rf[slot].x, e = rl[slot].y;
+        if (a >= e) continue;
+        double rem = r2;
+        if (D >= 2) {
+            const int dy = (D == 3 ? slot % 5 : slot) - 2;
+            const double ylo = g.origin[1] + (double)(c1 + dy) * s - slack, yhi = ylo + s + 2.0 * slack;
+            const double gy = fmax(0.0, fmax(ylo - (double)hi[1], (double)lo[1] - yhi));
+            rem -= gy * gy;
+        }
+        if (D >= 3) {
+            const int dz = slot / 5 - 2;
+            const double zlo = g.origin[2] + (double)(c2 + dz) * s - slack, zhi = zlo + s + 2.0 * slack;
+            const double gz = fmax(0.0, fmax(zlo - (double)hi[2], (double)lo[2] - zhi));
+            rem -= gz * gz;
+        }
+        if (uniform_int(rem < 0.0)) continue;
+        const double rx = sqrt(rem > 0.0 ? rem : 0.0) + slack;
+        const int xl = uniform_int((int)floor(((double)lo[0] - rx - g.origin[0]) * inv_s));
+        const int xh = uniform_int((int)floor(((double)hi[0] + rx - g.origin[0]) * inv_s));
+        while (a < e && cell_c[a] < xl) ++a;
+        while (e > a && cell_c[e - 1] > xh) --e;
+        const int end = cell_start[e];
+        int j = cell_start[a];
+        issued += end - j;
+        for (; j + 8 <= end; j += 8) cnt += count_block<D, 8>(qx, soa, npad, j, e2);
+        for (; j < end; ++j) cnt += count_block<D, 1>(qx, soa, npad, j, e2);
     }
     if (valid) nn_s[q] = cnt;
+    if (tests && lane == 0) atomicAdd(tests, (unsigned long long)(issued * tile.w));
 }
 
 // nn and core flags in input order (v_coord_nn, data_process.h:57), and core
@@ -484,7 +592,7 @@ static void grid_classify_d(Ctx &c) {
     c.vals2.ensure(n * 4);
     {
         Launch L(c.prof, s, "keys");
-        k_keys<D><<<nb, 256, 0, s>>>(c.coords, n, ig, c.offsets_d.as<int64_t>(), c.n_inst, c.lin_bits,
+        k_keys<D><<<nb, 256, 0, s>>>(c.coords, n, ig, c.offsets_d.as<int64_t>(), c.n_inst, c.lin_bits, c.sub_bits,
                                     c.keys.as<uint64_t>(), c.vals.as<uint32_t>());
     }
     bool alt = false;
@@ -504,15 +612,16 @@ static void grid_classify_d(Ctx &c) {
     c.cell_key.ensure(n * 8);
     c.misc.ensure(64);
     int32_t *d_total = c.misc.as<int32_t>();
+    unsigned long long *d_tests = reinterpret_cast<unsigned long long *>(c.misc.as<int32_t>() + 4);
     {
         Launch L(c.prof, s, "cell_flags");
-        k_cell_flags<<<nb, 256, 0, s>>>(keys, n, c.flag.as<int32_t>());
+        k_cell_flags<<<nb, 256, 0, s>>>(keys, n, c.sub_bits, c.flag.as<int32_t>());
     }
     scan_exclusive_i32(c.flag.as<int32_t>(), c.cid.as<int32_t>(), n, d_total, c.scratch, s, c.prof);
     {
         Launch L(c.prof, s, "cell_fill");
-        k_cell_fill<<<nb, 256, 0, s>>>(keys, c.flag.as<int32_t>(), c.cid.as<int32_t>(), n, c.cell_of.as<int32_t>(),
-                                       c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>());
+        k_cell_fill<<<nb, 256, 0, s>>>(keys, c.sub_bits, c.flag.as<int32_t>(), c.cid.as<int32_t>(), n,
+                                       c.cell_of.as<int32_t>(), c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>());
     }
     const int64_t npad = ceil_div(n, 64) * 64;
     c.soa.ensure((size_t)npad * D * 4);
@@ -538,35 +647,52 @@ static void grid_classify_d(Ctx &c) {
         k_row_insert<<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), nc, ig, c.lin_bits, c.row_keys.as<uint64_t>(),
                                          c.row_vals.as<int2>(), cap - 1);
     }
-    int rows = 1;
-    for (int k = 1; k < D; ++k) rows *= 2 * c.reach_max + 1;
-    c.nranges.ensure(nc * 4);
+    const int rows = c.rows_per_cell = D == 3 ? 25 : D == 2 ? 5 : 1;
     c.ranges.ensure((size_t)nc * rows * 8);
+    c.cell_c.ensure((size_t)nc * 3 * 4);
     c.tile_off.ensure((nc + 1) * 4);
     c.cid.ensure(nc * 4);
-    int32_t *work = c.cid.as<int32_t>();  // reused: tiles per cell
+    int32_t *work = c.cid.as<int32_t>();  // reused: label-pass tiles per cell
     {
         Launch L(c.prof, s, "stencil");
         k_stencil<D><<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), c.cell_start.as<int32_t>(), nc, ig, c.lin_bits,
                                          c.row_keys.as<uint64_t>(), c.row_vals.as<int2>(), cap - 1,
-                                         c.nranges.as<int32_t>(), c.ranges.as<int2>(), rows, work);
+                                         c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), work);
     }
     scan_exclusive_i32(work, c.tile_off.as<int32_t>(), nc, d_total, c.scratch, s, c.prof);
-    NDBX_HIP(hipMemcpyAsync(&h_total, d_total, 4, hipMemcpyDeviceToHost, s));
+    // count tiles over groups of cells of one row
+    c.gflag.ensure(nc * 4);
+    c.ctile_off.ensure((nc + 1) * 4);
+    {
+        Launch L(c.prof, s, "group_tiles");
+        k_group_flags<<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), c.cell_c.as<int32_t>(), nc, c.gflag.as<int32_t>());
+        k_group_tiles<<<ncb, 256, 0, s>>>(c.gflag.as<int32_t>(), c.cell_start.as<int32_t>(), nc, c.flag.as<int32_t>());
+    }
+    scan_exclusive_i32(c.flag.as<int32_t>(), c.ctile_off.as<int32_t>(), nc, d_total + 1, c.scratch, s, c.prof);
+    int32_t h_tot2[2] = {0, 0};
+    NDBX_HIP(hipMemcpyAsync(h_tot2, d_total, 8, hipMemcpyDeviceToHost, s));
     c.sync();
-    const int64_t nt = h_total;
-    c.n_tiles = nt;
+    const int64_t nt = h_tot2[0], nct = h_tot2[1];
+    c.n_tiles = nct;
+    c.n_label_tiles = nt;
     c.tiles.ensure(nt * 8);
+    c.ctiles.ensure(nct * 16);
     {
         Launch L(c.prof, s, "tiles");
         k_tiles<<<ncb, 256, 0, s>>>(c.cell_start.as<int32_t>(), c.tile_off.as<int32_t>(), nc, c.tiles.as<int2>());
+        k_group_tile_fill<<<ncb, 256, 0, s>>>(c.gflag.as<int32_t>(), c.cell_start.as<int32_t>(),
+                                              c.cell_of.as<int32_t>(), c.ctile_off.as<int32_t>(), nc,
+                                              c.ctiles.as<int4>());
     }
     c.nn_s.ensure(n * 4);
+    const bool stats = c.prof.on;
+    if (stats) fill_u64(reinterpret_cast<uint64_t *>(d_tests), 0, 1, s, c.prof);
     {
         Launch L(c.prof, s, "count");
-        k_count<D><<<(unsigned)ceil_div(nt, 4), 256, 0, s>>>(
-            c.tiles.as<int2>(), nt, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig, c.lin_bits,
-            c.nranges.as<int32_t>(), c.ranges.as<int2>(), rows, c.soa.as<float>(), npad, c.nn_s.as<int32_t>());
+        k_count<D><<<(unsigned)ceil_div(nct, 4), 256, 0, s>>>(
+            c.ctiles.as<int4>(), nct, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig, c.lin_bits,
+            c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), nc, c.soa.as<float>(), npad, c.nn_s.as<int32_t>(),
+            stats ? d_tests : nullptr);
     }
     c.core_s.ensure(n);
     {
@@ -574,6 +700,12 @@ static void grid_classify_d(Ctx &c) {
         k_classify_out<<<nb, 256, 0, s>>>(c.nn_s.as<int32_t>(), perm, c.cell_of.as<int32_t>(), c.cell_key.as<uint64_t>(),
                                           ig, c.lin_bits, n, c.nn.as<int32_t>(), c.core.as<uint8_t>(),
                                           c.core_s.as<uint8_t>());
+    }
+    if (stats) {
+        unsigned long long h_tests = 0;
+        NDBX_HIP(hipMemcpyAsync(&h_tests, d_tests, 8, hipMemcpyDeviceToHost, s));
+        c.sync();
+        c.pair_tests = (int64_t)h_tests;
     }
 }
 

@@ -45,22 +45,38 @@ __device__ void uf_unite(int32_t *parent, int32_t a, int32_t b) {
     }
 }
 
-// Per-cell core presence and lowest core id; core flags as int32 in input
-// order for the core-rank scan.
-__global__ __launch_bounds__(256) void k_core(const uint8_t *__restrict__ core_s, const uint32_t *__restrict__ perm,
-                                              const int32_t *__restrict__ cell_of, int64_t n,
-                                              int32_t *__restrict__ cell_core_min, int32_t *__restrict__ cell_has_core,
-                                              int32_t *__restrict__ core_i32) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const bool is_core = core_s[i];
-    const uint32_t p = perm[i];
-    core_i32[p] = is_core;
-    if (is_core) {
-        const int32_t c = cell_of[i];
-        atomicMin(cell_core_min + c, (int32_t)p);
-        cell_has_core[c] = 1;
+// Core flags as int32 in input order, for the core-rank scan; per cell, the
+// lowest input index among its cores.  The points of a cell are contiguous
+// in sorted order, so a wave whose points share one cell reduces first and
+// issues a single atomicMin.
+__global__ __launch_bounds__(256) void k_core_points(const uint8_t *__restrict__ core_s,
+                                                     const uint32_t *__restrict__ perm,
+                                                     const int32_t *__restrict__ cell_of, int64_t n,
+                                                     int32_t *__restrict__ core_i32,
+                                                     int32_t *__restrict__ cell_core_min) {
+    const int64_t wave_base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+    if (wave_base >= n) return;  // whole wave past the end
+    const int64_t i = wave_base + (threadIdx.x & 63);
+    const bool v = i < n;
+    const int64_t iv = v ? i : wave_base;
+    const bool is_core = v && core_s[iv];
+    const uint32_t p = perm[iv];
+    if (v) core_i32[p] = is_core;
+    const int32_t c = cell_of[iv];
+    int32_t val = is_core ? (int32_t)p : INT_MAX;
+    const int32_t c_first = __shfl(c, 0, 64);
+    if (__ballot(c != c_first) == 0) {
+        for (int o = 32; o >= 1; o >>= 1) val = min(val, __shfl_xor(val, o, 64));
+        if ((threadIdx.x & 63) == 0 && val != INT_MAX) atomicMin(cell_core_min + c, val);
+    } else if (is_core) {
+        atomicMin(cell_core_min + c, val);
     }
+}
+
+__global__ __launch_bounds__(256) void k_has_core(const int32_t *__restrict__ cell_core_min, int64_t nc,
+                                                  int32_t *__restrict__ cell_has_core) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nc) cell_has_core[c] = cell_core_min[c] != INT_MAX;
 }
 
 __global__ __launch_bounds__(256) void k_iota(int32_t *__restrict__ p, int64_t n) {
@@ -68,60 +84,210 @@ __global__ __launch_bounds__(256) void k_iota(int32_t *__restrict__ p, int64_t n
     if (i < n) p[i] = (int32_t)i;
 }
 
-// One wave per tile of a core cell A: for every neighbour cell B > A with a
-// core and a different root, test A's cores of this tile against B's cores
-// until one pair passes, then unite A and B.
+// Core-only copy of the sorted coordinates: a non-core point becomes NaN in
+// every dimension, so the float32 predicate rejects it without a flag test.
 template <int D>
-__global__ __launch_bounds__(256) void k_union(const int2 *__restrict__ tiles, int64_t n_tiles,
-                                               const int32_t *__restrict__ cell_start,
+__global__ __launch_bounds__(256) void k_soa_core(const float *__restrict__ soa, const uint8_t *__restrict__ core_s,
+                                                  int64_t n, int64_t npad, float *__restrict__ soa_core) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool c = core_s[i];
+#pragma unroll
+    for (int k = 0; k < D; ++k) soa_core[k * npad + i] = c ? soa[k * npad + i] : __builtin_nanf("");
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+    for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// Any of U candidates [j, j+U) within eps of this lane's query.
+template <int D, int U>
+__device__ __forceinline__ bool any_block(const float (&qx)[D], const float *__restrict__ soa, int64_t npad, int j,
+                                          float e2) {
+    float c[D][U];
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+#pragma unroll
+        for (int u = 0; u < U; ++u) c[k][u] = soa[k * npad + j + u];
+    bool hit = false;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        float len = 0.0f;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const float df = qx[k] - c[k][u];
+            len = __fadd_rn(len, __fmul_rn(df, df));
+        }
+        hit |= len <= e2;
+    }
+    return hit;
+}
+
+// True when some lane's query has a candidate in [b, e) within eps.
+template <int D>
+__device__ __forceinline__ bool wave_any_within(const float (&qx)[D], const float *__restrict__ soa, int64_t npad,
+                                                int b, int e, float e2) {
+    int j = b;
+    for (; j + 8 <= e; j += 8)
+        if (__ballot(any_block<D, 8>(qx, soa, npad, j, e2)) != 0) return true;
+    for (; j < e; ++j)
+        if (__ballot(any_block<D, 1>(qx, soa, npad, j, e2)) != 0) return true;
+    return false;
+}
+
+// Union of core cells, in two launches over all cells (one wave per cell A
+// with a core).
+//   phase 0: the face neighbours B = A+e_k (k < D) that hold a core, tested
+//            with early exit; this joins the bulk of every dense cluster.
+//   phase 1: every stencil neighbour B > A with a core, minus those already
+//            in A's set (finds run one lane per candidate), tested nearest
+//            first, and skipped when the box of A's current 64 cores is
+//            farther than the search radius from B's cell.
+// A test of (A, B) stops at the first core pair within eps; on a hit A and B
+// are united.  Either phase alone is correct; phase 0 makes phase 1 cheap.
+constexpr int kMaxStencilCells = 128;  // 25 rows x 5 cells for d <= 3
+
+template <int D>
+__global__ __launch_bounds__(256) void k_union(int64_t nc, int phase, const int32_t *__restrict__ cell_start,
                                                const uint64_t *__restrict__ cell_key,
                                                const InstGrid *__restrict__ ig, int lin_bits,
-                                               const int32_t *__restrict__ nranges, const int2 *__restrict__ ranges,
-                                               int rows_per_cell, const float *__restrict__ soa, int64_t npad,
-                                               const uint8_t *__restrict__ core_s,
-                                               const int32_t *__restrict__ cell_of,
+                                               const int2 *__restrict__ ranges, int rows_per_cell,
+                                               const int32_t *__restrict__ cell_c,
+                                               const float *__restrict__ soa_core, int64_t npad,
                                                const int32_t *__restrict__ cell_has_core,
                                                int32_t *__restrict__ parent) {
+    __shared__ int32_t cand[4][kMaxStencilCells];
+    __shared__ int32_t cand_root[4][kMaxStencilCells];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
-    if (t >= n_tiles) return;
-    const int lane = threadIdx.x & 63;
-    const int2 tile = tiles[t];
-    const int A = tile.x;
+    const int64_t A64 = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+    if (A64 >= nc) return;
+    const int A = (int)A64;
     if (!cell_has_core[A]) return;
-    const int q = tile.y + lane;
-    const bool active = q < cell_start[A + 1] && core_s[q];
-    if (__ballot(active) == 0) return;
-    const float e2 = ig[cell_key[A] >> lin_bits].e2;
-    float qx[D];
+    const int lane = threadIdx.x & 63;
+    const InstGrid &g = ig[cell_key[A] >> lin_bits];
+    const float e2 = g.e2;
+    const int a0 = cell_start[A], a1 = cell_start[A + 1];
+    const int ac0 = cell_c[A], ac1 = cell_c[nc + A], ac2 = cell_c[2 * nc + A];
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const int center = rows_per_cell / 2;  // slot of (dz, dy) = (0, 0)
+
+    // ---- candidate list
+    int n_cand = 0;
+    if (phase == 0) {
+        // lane k < D looks for the cell at +1 along dimension k
+        int B = -1;
+        if (lane < D) {
+            const int slot = lane == 0 ? center : (lane == 1 ? center + 1 : center + 5);
+            const int2 w = ranges[(int64_t)A * rows_per_cell + slot];
+            const int want0 = lane == 0 ? ac0 + 1 : ac0;
+            for (int b = w.x; b < w.y; ++b)
+                if (cell_c[b] == want0) B = b;
+            if (B >= 0 && !cell_has_core[B]) B = -1;
+        }
+        const uint64_t mask = __ballot(B >= 0);
+        if (B >= 0) cand[wave][__popcll(mask & lt)] = B;
+        n_cand = __popcll(mask);
+    } else {
+        int2 my = make_int2(0, 0);
+        if (lane < rows_per_cell) my = ranges[(int64_t)A * rows_per_cell + lane];
+        // gather B > A with a core, grouped nearest first by the squared gap
+        // in cells between A and B: sum over k of max(|dc_k| - 1, 0)^2
+        int Bs[5], cls[5];
 #pragma unroll
-    for (int k = 0; k < D; ++k) qx[k] = soa[k * npad + (active ? q : tile.y)];
-    const int nr = nranges[A];
-    const int2 *rr = ranges + (int64_t)A * rows_per_cell;
-    for (int r = 0; r < nr; ++r) {
-        const int2 rg = rr[r];
-        int j = rg.x;
-        while (j < rg.y) {
-            const int B = cell_of[j];
-            const int bend = cell_start[B + 1];
-            if (B > A && cell_has_core[B] && uf_find(parent, A) != uf_find(parent, B)) {
-                bool hit = false;
-                for (int jj = j; jj < bend; ++jj) {
-                    if (!core_s[jj]) continue;
-                    float len = 0.0f;
-#pragma unroll
-                    for (int k = 0; k < D; ++k) {
-                        const float df = qx[k] - soa[k * npad + jj];
-                        len = __fadd_rn(len, __fmul_rn(df, df));
-                    }
-                    if (__ballot(active && len <= e2) != 0) {
-                        hit = true;
-                        break;
-                    }
-                }
-                if (hit && lane == 0) uf_unite(parent, A, B);
+        for (int it = 0; it < 5; ++it) {
+            const int B = my.x + it;
+            const bool keep = B < my.y && B > A && cell_has_core[B];
+            Bs[it] = keep ? B : -1;
+            int cl = 0;
+            if (keep) {
+                const int d0 = abs(cell_c[B] - ac0), d1 = abs(cell_c[nc + B] - ac1), d2 = abs(cell_c[2 * nc + B] - ac2);
+                cl = (d0 > 1) + (d1 > 1) + (d2 > 1);
             }
-            j = bend;
+            cls[it] = cl;
+        }
+        for (int want = 0; want <= D; ++want) {
+#pragma unroll
+            for (int it = 0; it < 5; ++it) {
+                const bool take = Bs[it] >= 0 && cls[it] == want;
+                const uint64_t mask = __ballot(take);
+                if (take) cand[wave][n_cand + __popcll(mask & lt)] = Bs[it];
+                n_cand += __popcll(mask);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        // drop candidates already in A's set (one lane per candidate)
+        const int32_t root_a = uf_find(parent, A);
+        int kept = 0;
+        for (int base = 0; base < n_cand; base += 64) {
+            const int ci = base + lane;
+            int B = -1, rb = -1;
+            if (ci < n_cand) {
+                B = cand[wave][ci];
+                rb = uf_find(parent, B);
+                if (rb == root_a) B = -1;
+            }
+            __builtin_amdgcn_wave_barrier();
+            const uint64_t mask = __ballot(B >= 0);
+            if (B >= 0) {
+                cand[wave][kept + __popcll(mask & lt)] = B;
+                cand_root[wave][kept + __popcll(mask & lt)] = rb;
+            }
+            kept += __popcll(mask);
+        }
+        n_cand = kept;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (n_cand == 0) return;
+
+    // ---- tests
+    const double s = 1.0 / g.inv_s, slack = s * 9.5367431640625e-07;
+    const double r2 = (double)e2 * (1.0 + 3.814697265625e-06);
+    int32_t joined_root[4] = {-1, -1, -1, -1};  // roots met earlier in this wave (phase 1)
+    int n_joined = 0;
+    for (int q0 = a0; q0 < a1; q0 += 64) {
+        const int q = q0 + lane;
+        float qx[D], lo[D], hi[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) qx[k] = soa_core[k * npad + (q < a1 ? q : a0)];
+        if (q >= a1) qx[0] = __builtin_nanf("");
+        const bool is_core = qx[0] == qx[0];
+        if (__ballot(is_core) == 0) continue;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            lo[k] = wave_min(is_core ? qx[k] : __builtin_inff());
+            hi[k] = wave_max(is_core ? qx[k] : -__builtin_inff());
+        }
+        for (int ci = 0; ci < n_cand; ++ci) {
+            const int B = __builtin_amdgcn_readfirstlane(cand[wave][ci]);
+            if (B < 0) continue;  // joined already
+            if (phase == 1) {
+                const int rb = __builtin_amdgcn_readfirstlane(cand_root[wave][ci]);
+                bool seen = false;
+                for (int u = 0; u < n_joined && u < 4; ++u) seen |= joined_root[u] == rb;
+                if (seen) continue;
+            }
+            // box of this chunk's cores against B's cell box
+            double gap2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const double blo = g.origin[k] + (double)cell_c[k * nc + B] * s - slack, bhi = blo + s + 2.0 * slack;
+                const double gk = fmax(0.0, fmax(blo - (double)hi[k], (double)lo[k] - bhi));
+                gap2 += gk * gk;
+            }
+            if (__builtin_amdgcn_readfirstlane(gap2 > r2)) continue;
+            if (wave_any_within<D>(qx, soa_core, npad, cell_start[B], cell_start[B + 1], e2)) {
+                if (lane == 0) {
+                    cand[wave][ci] = -1;
+                    uf_unite(parent, A, B);
+                }
+                if (phase == 1 && n_joined < 4) joined_root[n_joined++] = __builtin_amdgcn_readfirstlane(cand_root[wave][ci]);
+                __builtin_amdgcn_wave_barrier();
+            }
         }
     }
 }
@@ -171,10 +337,10 @@ __global__ __launch_bounds__(256) void k_border(const int2 *__restrict__ tiles, 
                                                 const int32_t *__restrict__ cell_start,
                                                 const uint64_t *__restrict__ cell_key,
                                                 const InstGrid *__restrict__ ig, int lin_bits,
-                                                const int32_t *__restrict__ nranges, const int2 *__restrict__ ranges,
-                                                int rows_per_cell, const float *__restrict__ soa, int64_t npad,
+                                                const int2 *__restrict__ ranges,
+                                                int rows_per_cell, const float *__restrict__ soa,
+                                                const float *__restrict__ soa_core, int64_t npad,
                                                 const uint8_t *__restrict__ core_s,
-                                                const int32_t *__restrict__ cell_of,
                                                 const int32_t *__restrict__ cell_label,
                                                 const uint32_t *__restrict__ perm, int32_t *__restrict__ labels) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -188,38 +354,37 @@ __global__ __launch_bounds__(256) void k_border(const int2 *__restrict__ tiles, 
     const bool is_core = valid && core_s[q];
     const int32_t LA = cell_label[A];
     int32_t best = LA;  // NO_CLUSTER when A has no core
-    bool need = valid && !is_core;
+    const bool need = valid && !is_core;
     if (__ballot(need) != 0) {
         const float e2 = ig[cell_key[A] >> lin_bits].e2;
         float qx[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) qx[k] = soa[k * npad + (valid ? q : tile.y)];
-        const int nr = nranges[A];
+        const int nr = rows_per_cell;
         const int2 *rr = ranges + (int64_t)A * rows_per_cell;
         for (int r = 0; r < nr; ++r) {
-            const int2 rg = rr[r];
-            int j = rg.x;
-            while (j < rg.y) {
-                const int B = cell_of[j];
-                const int bend = cell_start[B + 1];
+            const int2 cr = rr[r];
+            for (int B = cr.x; B < cr.y; ++B) {
                 const int32_t LB = cell_label[B];
-                if (B != A && LB != NDBX_NO_CLUSTER) {
-                    bool want = need && LB < best;
-                    for (int jj = j; jj < bend && __ballot(want) != 0; ++jj) {
-                        if (!core_s[jj]) continue;
-                        float len = 0.0f;
+                // only lanes that would improve take part; others see NaN
+                const bool want = need && LB < best;
+                if (B == A || LB == NDBX_NO_CLUSTER || __ballot(want) == 0) continue;
+                float qw[D];
 #pragma unroll
-                        for (int k = 0; k < D; ++k) {
-                            const float df = qx[k] - soa[k * npad + jj];
-                            len = __fadd_rn(len, __fmul_rn(df, df));
-                        }
-                        if (want && len <= e2) {
-                            best = LB;
-                            want = false;
-                        }
-                    }
+                for (int k = 0; k < D; ++k) qw[k] = want ? qx[k] : __builtin_nanf("");
+                const int b1 = cell_start[B + 1];
+                bool got = false;
+                int j = cell_start[B];
+                bool open = true;  // some wanting lane has no hit yet (wave-uniform)
+                for (; j + 8 <= b1 && open; j += 8) {
+                    got |= any_block<D, 8>(qw, soa_core, npad, j, e2);
+                    open = __ballot(want && !got) != 0;
                 }
-                j = bend;
+                for (; j < b1 && open; ++j) {
+                    got |= any_block<D, 1>(qw, soa_core, npad, j, e2);
+                    open = __ballot(want && !got) != 0;
+                }
+                if (got) best = LB;
             }
         }
     }
@@ -227,35 +392,67 @@ __global__ __launch_bounds__(256) void k_border(const int2 *__restrict__ tiles, 
 }
 
 // Per-instance (cores, noise, clusters) from input-order flags and labels;
-// clusters = core cells that are roots.
+// clusters = core cells that are roots.  A wave whose points all lie in one
+// instance adds its sums with one atomic per counter.
 __global__ __launch_bounds__(256) void k_meta_points(const uint8_t *__restrict__ core, const int32_t *__restrict__ labels,
                                                      const int64_t *__restrict__ off, int n_inst, int64_t n,
                                                      int32_t *__restrict__ meta) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool v = i < n;
-    const int c = v && core[i];
-    const int z = v && labels[i] == NDBX_NO_CLUSTER;
-    if (n_inst == 1) {
-        int a = c, b = z;
-        for (int o = 32; o >= 1; o >>= 1) {
-            a += __shfl_xor(a, o, 64);
-            b += __shfl_xor(b, o, 64);
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int a_acc = 0, b_acc = 0;  // single-instance accumulators
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
+        const int64_t i = base + lane;
+        const bool v = i < n;
+        const int c = v && core[i];
+        const int z = v && labels[i] == NDBX_NO_CLUSTER;
+        if (n_inst == 1) {
+            a_acc += c;
+            b_acc += z;
+            continue;
         }
-        if ((threadIdx.x & 63) == 0) {
+        int inst = 0;
+        if (v) {
+            int lo = 0, hi = n_inst - 1;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (off[mid] <= i) lo = mid;
+                else hi = mid - 1;
+            }
+            inst = lo;
+        }
+        const int first = __shfl(inst, 0, 64);
+        if (__ballot(v && inst != first) == 0) {
+            int a = c, b = z;
+            for (int o = 32; o >= 1; o >>= 1) {
+                a += __shfl_xor(a, o, 64);
+                b += __shfl_xor(b, o, 64);
+            }
+            if (lane == 0) {
+                if (a) atomicAdd(meta + 3 * first + 0, a);
+                if (b) atomicAdd(meta + 3 * first + 1, b);
+            }
+        } else {
+            if (c) atomicAdd(meta + 3 * inst + 0, 1);
+            if (z) atomicAdd(meta + 3 * inst + 1, 1);
+        }
+    }
+    if (n_inst == 1) {
+        __shared__ int sa[4], sb[4];
+        for (int o = 32; o >= 1; o >>= 1) {
+            a_acc += __shfl_xor(a_acc, o, 64);
+            b_acc += __shfl_xor(b_acc, o, 64);
+        }
+        if (lane == 0) {
+            sa[threadIdx.x >> 6] = a_acc;
+            sb[threadIdx.x >> 6] = b_acc;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int a = sa[0] + sa[1] + sa[2] + sa[3], b = sb[0] + sb[1] + sb[2] + sb[3];
             if (a) atomicAdd(meta + 0, a);
             if (b) atomicAdd(meta + 1, b);
         }
-        return;
     }
-    if (!v) return;
-    int lo = 0, hi = n_inst - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= i) lo = mid;
-        else hi = mid - 1;
-    }
-    if (c) atomicAdd(meta + 3 * lo + 0, 1);
-    if (z) atomicAdd(meta + 3 * lo + 1, 1);
 }
 
 __global__ __launch_bounds__(256) void k_meta_cells(const int32_t *__restrict__ cell_has_core,
@@ -269,12 +466,11 @@ __global__ __launch_bounds__(256) void k_meta_cells(const int32_t *__restrict__ 
 
 template <int D>
 static void grid_label_d(Ctx &c) {
-    const int64_t n = c.n, nc = c.n_cells, nt = c.n_tiles;
+    const int64_t n = c.n, nc = c.n_cells, nt = c.n_label_tiles;
     const hipStream_t s = c.stream;
     const int nb = (int)ceil_div(n, 256), ncb = (int)ceil_div(nc, 256);
     InstGrid *ig = c.inst_d.as<InstGrid>();
-    int rows = 1;
-    for (int k = 1; k < D; ++k) rows *= 2 * c.reach_max + 1;
+    const int rows = c.rows_per_cell;
     const int64_t npad = ceil_div(n, 64) * 64;
 
     c.cell_core_min.ensure(nc * 4);
@@ -282,12 +478,11 @@ static void grid_label_d(Ctx &c) {
     c.rank.ensure((n + 1) * 4);
     c.label_s.ensure(n * 4);  // core flags as int32, input order
     fill_i32(c.cell_core_min.as<int32_t>(), INT_MAX, nc, s, c.prof);
-    fill_i32(c.cell_has_core.as<int32_t>(), 0, nc, s, c.prof);
     {
         Launch L(c.prof, s, "core");
-        k_core<<<nb, 256, 0, s>>>(c.core_s.as<uint8_t>(), c.vals.as<uint32_t>(), c.cell_of.as<int32_t>(), n,
-                                  c.cell_core_min.as<int32_t>(), c.cell_has_core.as<int32_t>(),
-                                  c.label_s.as<int32_t>());
+        k_core_points<<<nb, 256, 0, s>>>(c.core_s.as<uint8_t>(), c.vals.as<uint32_t>(), c.cell_of.as<int32_t>(), n,
+                                         c.label_s.as<int32_t>(), c.cell_core_min.as<int32_t>());
+        k_has_core<<<ncb, 256, 0, s>>>(c.cell_core_min.as<int32_t>(), nc, c.cell_has_core.as<int32_t>());
     }
     int32_t *d_total = c.misc.as<int32_t>();
     scan_exclusive_i32(c.label_s.as<int32_t>(), c.rank.as<int32_t>(), n, d_total, c.scratch, s, c.prof);
@@ -304,12 +499,18 @@ static void grid_label_d(Ctx &c) {
     }
     fill_i32(c.comp_min.as<int32_t>(), INT_MAX, nc, s, c.prof);
     const unsigned tb = (unsigned)ceil_div(nt, 4);
+    c.soa_core.ensure((size_t)npad * D * 4);
     {
-        Launch L(c.prof, s, "union");
-        k_union<D><<<tb, 256, 0, s>>>(c.tiles.as<int2>(), nt, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(),
-                                      ig, c.lin_bits, c.nranges.as<int32_t>(), c.ranges.as<int2>(), rows,
-                                      c.soa.as<float>(), npad, c.core_s.as<uint8_t>(), c.cell_of.as<int32_t>(),
-                                      c.cell_has_core.as<int32_t>(), c.parent.as<int32_t>());
+        Launch L(c.prof, s, "soa_core");
+        k_soa_core<D><<<nb, 256, 0, s>>>(c.soa.as<float>(), c.core_s.as<uint8_t>(), n, npad, c.soa_core.as<float>());
+    }
+    for (int phase = 0; phase < 2; ++phase) {
+        Launch L(c.prof, s, phase == 0 ? "union_faces" : "union");
+        k_union<D><<<(unsigned)ceil_div(nc, 4), 256, 0, s>>>(nc, phase, c.cell_start.as<int32_t>(),
+                                                             c.cell_key.as<uint64_t>(), ig, c.lin_bits,
+                                                             c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(),
+                                                             c.soa_core.as<float>(), npad,
+                                                             c.cell_has_core.as<int32_t>(), c.parent.as<int32_t>());
     }
     {
         Launch L(c.prof, s, "flatten");
@@ -325,14 +526,14 @@ static void grid_label_d(Ctx &c) {
     {
         Launch L(c.prof, s, "border");
         k_border<D><<<tb, 256, 0, s>>>(c.tiles.as<int2>(), nt, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(),
-                                       ig, c.lin_bits, c.nranges.as<int32_t>(), c.ranges.as<int2>(), rows,
-                                       c.soa.as<float>(), npad, c.core_s.as<uint8_t>(), c.cell_of.as<int32_t>(),
+                                       ig, c.lin_bits, c.ranges.as<int2>(), rows,
+                                       c.soa.as<float>(), c.soa_core.as<float>(), npad, c.core_s.as<uint8_t>(),
                                        c.cell_label.as<int32_t>(), c.vals.as<uint32_t>(), c.labels.as<int32_t>());
     }
     fill_i32(c.meta.as<int32_t>(), 0, 3 * c.n_inst, s, c.prof);
     {
         Launch L(c.prof, s, "meta");
-        k_meta_points<<<nb, 256, 0, s>>>(c.core.as<uint8_t>(), c.labels.as<int32_t>(), c.offsets_d.as<int64_t>(),
+        k_meta_points<<<(unsigned)std::min<int64_t>(ceil_div(n, 256), 2048), 256, 0, s>>>(c.core.as<uint8_t>(), c.labels.as<int32_t>(), c.offsets_d.as<int64_t>(),
                                          c.n_inst, n, c.meta.as<int32_t>());
         k_meta_cells<<<ncb, 256, 0, s>>>(c.cell_has_core.as<int32_t>(), c.cell_key.as<uint64_t>(), c.lin_bits,
                                          c.parent.as<int32_t>(), nc, c.meta.as<int32_t>());
