@@ -96,6 +96,9 @@ def main():
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD)
     ap.add_argument("--n", type=int, default=None, help="smaller instance of the same workload shape")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first run one device-to-device copy of BYTES (a known-size kernel for "
+                         "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs one GPU: every box in this setup has one MI355X, and the multi-GPU "
@@ -113,6 +116,12 @@ def main():
     n, d = x.shape
     eps_text = {"blobs2d_100k": "0.5", "blobs3d_10m": "1.5", "skew2d_100m": "0.3"}.get(args.workload, repr(float(eps)))
 
+    if args.copy_calibration:
+        src = torch.ones(args.copy_calibration // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     ctx = ndbx.Context(x, eps, m, flags=ndbx.PROFILE)
     for _ in range(args.warmup):
         ctx.classify().label()
