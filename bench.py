@@ -88,6 +88,16 @@ def traffic_from_profiles(kernel, workload):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def Ctx_stats(ctx):
+    import ndbx
+    return ndbx.Context.stats(ctx)
+
+
+def ctx_path(ctx):
+    import ndbx
+    return ndbx.Context.path(ctx)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +105,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD)
     ap.add_argument("--n", type=int, default=None, help="smaller instance of the same workload shape")
+    ap.add_argument("--n-inst", type=int, default=4096, help="instances of the batched workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one device-to-device copy of BYTES (a known-size kernel for "
@@ -112,35 +123,75 @@ def main():
 
     if not ndbx.device_available():
         raise RuntimeError("no HIP device")
-    x, eps, m = synth.make(args.workload, n=args.n)
-    n, d = x.shape
-    eps_text = {"blobs2d_100k": "0.5", "blobs3d_10m": "1.5", "skew2d_100m": "0.3"}.get(args.workload, repr(float(eps)))
-
     if args.copy_calibration:
         src = torch.ones(args.copy_calibration // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    ctx = ndbx.Context(x, eps, m, flags=ndbx.PROFILE)
+
+    extra = {}
+    if args.workload.startswith("batch3d"):
+        # many independent instances: one batched run vs the plain form (each
+        # instance its own context), both from device-resident points
+        n_inst = args.n_inst
+        x, off, eps_arr = synth.batch3d(n_inst=n_inst)
+        n, d = x.shape
+        m = 8
+        mins = np.full(n_inst, m, np.int32)
+        ctx = ndbx.Batch(x, off, eps_arr, mins, flags=ndbx.PROFILE)
+        step = lambda: (ctx.run(), ctx.meta())
+        eps = float(eps_arr.mean())
+        eps_text = None
+        x_dev = torch.from_numpy(x).to("cuda")
+        plain = [ndbx.Context(None, eps_arr[i], m, device_ptr=x_dev.data_ptr() + int(off[i]) * 4 * d,
+                              n=int(off[i + 1] - off[i]), d=d) for i in range(n_inst)]
+        for c in plain:  # warm: allocate every context's buffers
+            c.classify().label()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(max(1, args.steps // 5)):
+            for c in plain:
+                c.classify().label()
+                c.meta()
+        torch.cuda.synchronize()
+        plain_ms = (time.perf_counter() - t0) * 1e3 / max(1, args.steps // 5)
+        # the plain form must give the batched form's per-instance results
+        ctx.run()
+        bmeta = ctx.meta()
+        for i in range(0, n_inst, max(1, n_inst // 64)):
+            assert tuple(bmeta[i]) == plain[i].meta(), f"instance {i} differs"
+        for c in plain:
+            c.close()
+        extra["plain_gpu"] = {"ms_per_step": plain_ms, "value": n / (plain_ms * 1e-3),
+                              "form": "one context per instance, run one after another"}
+        extra["instances"] = n_inst
+    else:
+        x, eps, m = synth.make(args.workload, n=args.n)
+        n, d = x.shape
+        eps_text = {"blobs2d_100k": "0.5", "blobs3d_10m": "1.5", "skew2d_100m": "0.3"}.get(args.workload,
+                                                                                             repr(float(eps)))
+        ctx = ndbx.Context(x, eps, m, flags=ndbx.PROFILE)
+        step = lambda: (ctx.classify().label(), ctx.meta())
+
     for _ in range(args.warmup):
-        ctx.classify().label()
-        ctx.meta()
+        step()
     ctx.profile(reset=True)
     torch.cuda.synchronize()
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        ctx.classify().label()
-        meta = ctx.meta()
+        _, meta = step()
     ctx.sync()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     prof = ctx.profile()
-    stats = ctx.stats()
-    path = ctx.path()
+    stats = Ctx_stats(ctx)
+    path = ctx_path(ctx)
     nn = ctx.nn()
+    if isinstance(meta, np.ndarray):
+        meta = [int(v) for v in meta.sum(0)]
     ctx.close()
 
     kernels = {k: {"ms_avg": v[0] / max(v[1], 1), "launches_per_step": v[1] / args.steps}
@@ -163,7 +214,7 @@ def main():
                 tf = flops / avg_s / 1e12
                 roofline["valu"] = {"achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                                     "frac": tf / FP32_PEAK_TFLOPS, "flops": flops}
-    cpu = None if args.no_cpu_baseline else cpu_baseline(x, eps_text, m, n)
+    cpu = None if (args.no_cpu_baseline or eps_text is None) else cpu_baseline(x, eps_text, m, n)
 
     out = {
         "metric": METRIC,
@@ -184,6 +235,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
     }
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 

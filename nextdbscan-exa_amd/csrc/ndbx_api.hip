@@ -1,5 +1,7 @@
 // C ABI of the ndbx library: contexts, argument checks, status codes.
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <new>
 
 #include "ndbx_internal.h"
@@ -20,7 +22,7 @@ Ctx::~Ctx() {
                       &keys2, &vals2, &flag, &cid, &cell_of, &cell_start, &cell_key, &soa, &row_keys, &row_vals,
                       &nranges, &ranges, &tile_off, &tiles, &nn_s, &core_s, &cell_core_min, &cell_has_core,
                       &parent, &comp_min, &cell_label, &label_s, &rank, &misc, &scratch, &scratch2, &b_parent,
-                      &b_rootmin, &soa_core, &cell_c, &gflag, &ctile_off, &ctiles};
+                      &b_rootmin, &soa_core, &cell_c, &gflag, &ctile_off, &ctiles, &tile_tests};
     for (auto *b : bufs) b->release();
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -53,6 +55,7 @@ static void open_common(Ctx &c, const float *coords, int64_t n_total, uint32_t f
     NDBX_HIP(hipGetDevice(&c.device));
     NDBX_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     c.prof.on = (flags & NDBX_PROFILE) != 0;
+    if (const char *gc = std::getenv("NDBX_GROUP_CELLS")) c.group_cells = std::max(1, std::atoi(gc));
     c.n = n_total;
     c.flags = flags;
     const size_t bytes = (size_t)n_total * (size_t)c.d * sizeof(float);

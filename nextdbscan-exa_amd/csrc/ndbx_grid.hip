@@ -12,6 +12,7 @@
 #include <cmath>
 
 #include "ndbx_internal.h"
+#include "ndbx_tile.h"
 
 namespace ndbx {
 
@@ -323,9 +324,12 @@ __global__ __launch_bounds__(256) void k_stencil(const uint64_t *__restrict__ ce
                                                  const InstGrid *__restrict__ ig, int lin_bits,
                                                  const uint64_t *__restrict__ tkeys, const int2 *__restrict__ tvals,
                                                  uint64_t mask, int2 *__restrict__ ranges, int rows_per_cell,
-                                                 int32_t *__restrict__ cell_c, int32_t *__restrict__ work) {
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
+                                                 int32_t *__restrict__ cell_c) {
+    // one thread per (cell, stencil row)
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= nc * rows_per_cell) return;
+    const int64_t c = tid / rows_per_cell;
+    const int slot = (int)(tid - c * rows_per_cell);
     const uint64_t key = cell_key[c];
     const uint64_t inst = key >> lin_bits;
     const InstGrid &g = ig[inst];
@@ -335,70 +339,54 @@ __global__ __launch_bounds__(256) void k_stencil(const uint64_t *__restrict__ ce
     lin /= (uint64_t)G0;
     const int64_t c1 = D >= 2 ? (int64_t)(lin % (uint64_t)G1) : 0;
     const int64_t c2 = D >= 3 ? (int64_t)(lin / (uint64_t)G1) : 0;
-    cell_c[c] = (int32_t)c0;
-    cell_c[nc + c] = (int32_t)c1;
-    cell_c[2 * nc + c] = (int32_t)c2;
+    if (slot == 0) {
+        cell_c[c] = (int32_t)c0;
+        cell_c[nc + c] = (int32_t)c1;
+        cell_c[2 * nc + c] = (int32_t)c2;
+    }
     const int k = g.reach;
+    const int dz = D >= 3 ? slot / (2 * k + 1) - k : 0;
+    const int dy = D >= 3 ? slot % (2 * k + 1) - k : (D == 2 ? slot - k : 0);
     const int64_t lo0 = c0 - k < 0 ? 0 : c0 - k;
     const int64_t hi0 = c0 + k >= G0 ? G0 - 1 : c0 + k;
-    const uint64_t ibase = inst << lin_bits;
-    const int kz = D >= 3 ? k : 0, ky = D >= 2 ? k : 0;
-    int slot = 0;
-    for (int dz = -kz; dz <= kz; ++dz) {
-        for (int dy = -ky; dy <= ky; ++dy, ++slot) {
-            int2 out = make_int2(0, 0);
-            const int64_t z = c2 + dz, y = c1 + dy;
-            int2 rv;
-            if (z >= 0 && z < G2 && y >= 0 && y < G1) {
-                const uint64_t rk = ibase | (uint64_t)((z * G1 + y) * G0);
-                if (row_lookup(tkeys, tvals, mask, rk, &rv)) {
-                    int a = rv.x, b = rv.y;  // first cell with key >= rk+lo0
-                    const uint64_t klo = rk + (uint64_t)lo0, khi = rk + (uint64_t)hi0;
-                    while (a < b) {
-                        int mid = (a + b) >> 1;
-                        if (cell_key[mid] < klo) a = mid + 1;
-                        else b = mid;
-                    }
-                    int e = a, f = rv.y;
-                    while (e < f) {  // first cell with key > khi
-                        int mid = (e + f) >> 1;
-                        if (cell_key[mid] <= khi) e = mid + 1;
-                        else f = mid;
-                    }
-                    out = make_int2(a, e);
-                }
+    int2 out = make_int2(0, 0);
+    const int64_t z = c2 + dz, y = c1 + dy;
+    int2 rv;
+    if (z >= 0 && z < G2 && y >= 0 && y < G1) {
+        const uint64_t rk = (inst << lin_bits) | (uint64_t)((z * G1 + y) * G0);
+        if (row_lookup(tkeys, tvals, mask, rk, &rv)) {
+            int a = rv.x, b = rv.y;  // first cell with key >= rk+lo0
+            const uint64_t klo = rk + (uint64_t)lo0, khi = rk + (uint64_t)hi0;
+            while (a < b) {
+                int mid = (a + b) >> 1;
+                if (cell_key[mid] < klo) a = mid + 1;
+                else b = mid;
             }
-            ranges[c * rows_per_cell + slot] = out;
+            int e = a, f = rv.y;
+            while (e < f) {  // first cell with key > khi
+                int mid = (e + f) >> 1;
+                if (cell_key[mid] <= khi) e = mid + 1;
+                else f = mid;
+            }
+            out = make_int2(a, e);
         }
     }
-    const int32_t cnt = cell_start[c + 1] - cell_start[c];
-    work[c] = (cnt + kTile - 1) / kTile;  // label-pass tiles of this cell
+    ranges[c * rows_per_cell + slot] = out;
 }
 
-__global__ __launch_bounds__(256) void k_tiles(const int32_t *__restrict__ cell_start,
-                                               const int32_t *__restrict__ tile_off, int64_t nc,
-                                               int2 *__restrict__ tiles) {
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    const int32_t b = cell_start[c], e = cell_start[c + 1];
-    int32_t t = tile_off[c];
-    for (int32_t q = b; q < e; q += kTile) tiles[t++] = make_int2((int32_t)c, q);
-}
-
-// Counting groups: the cells of one row whose c0 share c0 / kGroupCells; the
+// Counting groups: the cells of one row whose c0 share c0 / group_cells; the
 // group's points are contiguous in sorted order and are cut into tiles of at
 // most 64 points (one wave each).
-constexpr int kGroupCells = 2;
 
 __global__ __launch_bounds__(256) void k_group_flags(const uint64_t *__restrict__ cell_key,
                                                      const int32_t *__restrict__ cell_c, int64_t nc,
-                                                     int32_t *__restrict__ gflag) {
+                                                     int group_cells, int32_t *__restrict__ gflag) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     bool f = c == 0;
     if (!f) {
         const uint64_t rk = cell_key[c] - (uint64_t)cell_c[c], rk_prev = cell_key[c - 1] - (uint64_t)cell_c[c - 1];
-        f = rk != rk_prev || cell_c[c] / kGroupCells != cell_c[c - 1] / kGroupCells;
+        f = rk != rk_prev || cell_c[c] / group_cells != cell_c[c - 1] / group_cells;
     }
     gflag[c] = f;
 }
@@ -464,25 +452,15 @@ __device__ __forceinline__ int32_t count_block(const float (&qx)[D], const float
     return cnt;
 }
 
-__device__ __forceinline__ float wave_min(float v) {
-    for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
-__device__ __forceinline__ float wave_max(float v) {
-    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
-__device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+constexpr int kStage = 256;  // candidates staged in LDS per wave (sparse tiles)
 
-// One wave per count tile (<= 64 points of a run of cells of one row).  For
-// every stencil row the tile's bounding box first narrows the window: a row
-// whose perpendicular gap exceeds the search radius is skipped, and along
-// dimension 0 only cells that can hold a point within the radius of the box
-// are kept.  The radius is sqrt(e2 (1 + 2^-18)), above any distance the
-// float32 predicate can accept, and cell bounds are widened by 2^-20 of a
-// cell, so no neighbour is dropped.  The candidate loop is wave-uniform
-// (scalar loads) and each lane tests its own point with the reference's
-// arithmetic (data_process.cpp:443-447): ascending d, no contraction, '<='.
+// One wave per count tile (<= 64 points of a run of cells of one row).  The
+// lanes first compute the candidate window of every stencil row in parallel
+// (tile_window: rows and cells out of reach of the tile's bounding box are
+// dropped), then the wave walks the non-empty windows.  The candidate loop is
+// wave-uniform (scalar loads) and each lane tests its own point with the
+// reference's arithmetic (data_process.cpp:443-447): ascending d, no
+// contraction, '<='.
 template <int D>
 __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, int64_t n_tiles,
                                                const int32_t *__restrict__ cell_start,
@@ -492,7 +470,7 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
                                                const int32_t *__restrict__ cell_c, int64_t nc,
                                                const float *__restrict__ soa, int64_t npad,
                                                int32_t *__restrict__ nn_s,
-                                               unsigned long long *__restrict__ tests) {
+                                               int64_t *__restrict__ tests) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
     if (t >= n_tiles) return;
@@ -510,43 +488,95 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
     }
     const InstGrid &g = ig[cell_key[cf] >> lin_bits];
     const float e2 = g.e2;
-    const double inv_s = g.inv_s, s = 1.0 / inv_s, slack = s * 9.5367431640625e-07;  // 2^-20 cell
-    const double r2 = (double)e2 * (1.0 + 3.814697265625e-06);                        // (1 + 2^-18)
-    const int c1 = cell_c[nc + cf], c2 = cell_c[2 * nc + cf];
+    const int2 w = tile_window<D>(lane, rows_per_cell, cf, cl, lo, hi, g, cell_c[nc + cf], cell_c[2 * nc + cf],
+                                  ranges, cell_c, nc);
+    const int ps = w.x < w.y ? cell_start[w.x] : 0, pe = w.x < w.y ? cell_start[w.y] : 0;
+    uint64_t rows = __ballot(pe > ps);
     int32_t cnt = 0;
     int64_t issued = 0;
-    const int2 *rf = ranges + (int64_t)cf * rows_per_cell;
-    const int2 *rl = ranges + (int64_t)cl * rows_per_cell;
-    for (int slot = 0; slot < rows_per_cell; ++slot) {
-        int a = rf[slot].x, e = rl[slot].y;
-        if (a >= e) continue;
-        double rem = r2;
-        if (D >= 2) {
-            const int dy = (D == 3 ? slot % 5 : slot) - 2;
-            const double ylo = g.origin[1] + (double)(c1 + dy) * s - slack, yhi = ylo + s + 2.0 * slack;
-            const double gy = fmax(0.0, fmax(ylo - (double)hi[1], (double)lo[1] - yhi));
-            rem -= gy * gy;
+    if (tile.w > 16) {
+        // wave-uniform candidates: every lane tests its own query
+        while (rows) {
+            const int slot = __builtin_ctzll(rows);
+            rows &= rows - 1;
+            const int end = __builtin_amdgcn_readlane(pe, slot);
+            int j = __builtin_amdgcn_readlane(ps, slot);
+            issued += end - j;
+            for (; j + 8 <= end; j += 8) cnt += count_block<D, 8>(qx, soa, npad, j, e2);
+            for (; j < end; ++j) cnt += count_block<D, 1>(qx, soa, npad, j, e2);
         }
-        if (D >= 3) {
-            const int dz = slot / 5 - 2;
-            const double zlo = g.origin[2] + (double)(c2 + dz) * s - slack, zhi = zlo + s + 2.0 * slack;
-            const double gz = fmax(0.0, fmax(zlo - (double)hi[2], (double)lo[2] - zhi));
-            rem -= gz * gz;
-        }
-        if (uniform_int(rem < 0.0)) continue;
-        const double rx = sqrt(rem > 0.0 ? rem : 0.0) + slack;
-        const int xl = uniform_int((int)floor(((double)lo[0] - rx - g.origin[0]) * inv_s));
-        const int xh = uniform_int((int)floor(((double)hi[0] + rx - g.origin[0]) * inv_s));
-        while (a < e && cell_c[a] < xl) ++a;
-        while (e > a && cell_c[e - 1] > xh) --e;
-        const int end = cell_start[e];
-        int j = cell_start[a];
-        issued += end - j;
-        for (; j + 8 <= end; j += 8) cnt += count_block<D, 8>(qx, soa, npad, j, e2);
-        for (; j < end; ++j) cnt += count_block<D, 1>(qx, soa, npad, j, e2);
+        if (valid) nn_s[q] = cnt;
+        if (tests && lane == 0) tests[t] = issued * tile.w;
+        return;
     }
-    if (valid) nn_s[q] = cnt;
-    if (tests && lane == 0) atomicAdd(tests, (unsigned long long)(issued * tile.w));
+    // Few queries (sparse tiles): the windows are flattened into one candidate
+    // sequence (wave scan of their lengths), staged through LDS 256 at a time
+    // with independent loads, and split over lanes = (query qi, slice sub).
+    __shared__ float stage[4][D][kStage];
+    const int len = pe - ps;
+    int incl = len;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t2 = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t2;
+    }
+    const int pre = incl - len;
+    const int total = __builtin_amdgcn_readlane(incl, 63);
+    const int qb = tile.w <= 1 ? 0 : tile.w <= 2 ? 1 : tile.w <= 4 ? 2 : tile.w <= 8 ? 3 : 4;
+    const int qi = lane & ((1 << qb) - 1), sub = lane >> qb, nsub = 64 >> qb;
+    const bool qvalid = qi < tile.w;
+    float my[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) my[k] = __shfl(qx[k], qvalid ? qi : 0, 64);
+    for (int cb = 0; cb < total; cb += kStage) {
+        const int n_here = total - cb < kStage ? total - cb : kStage;
+#pragma unroll
+        for (int u = 0; u < kStage / 64; ++u) {
+            const int gi = cb + u * 64 + lane;
+            const int gc = gi < total ? gi : total - 1;  // all lanes take part in the shuffles
+            int lo_w = 0, hi_w = 63;  // last lane whose window starts at or before gc
+#pragma unroll
+            for (int it = 0; it < 6; ++it) {
+                const int mid = (lo_w + hi_w + 1) >> 1;
+                if (__shfl(pre, mid, 64) <= gc) lo_w = mid;
+                else hi_w = mid - 1;
+            }
+            const int j = __shfl(ps, lo_w, 64) + gc - __shfl(pre, lo_w, 64);
+            if (gi < total) {
+#pragma unroll
+                for (int k = 0; k < D; ++k) stage[wave][k][u * 64 + lane] = soa[k * npad + j];
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (int i = sub; i < n_here; i += nsub) {
+            float l2 = 0.0f;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const float df = my[k] - stage[wave][k][i];
+                l2 = __fadd_rn(l2, __fmul_rn(df, df));
+            }
+            cnt += (l2 <= e2) ? 1 : 0;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    issued = total;
+    // sum the slices of each query (lanes qi, qi + 2^qb, ...)
+    for (int o = 32; o >= (1 << qb); o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+    if (sub == 0 && qvalid) nn_s[tile.z + qi] = cnt;
+    if (tests && lane == 0) tests[t] = issued * tile.w;
+}
+
+// Sum of int64 values (pair-test statistics): one atomic per block.
+__global__ __launch_bounds__(256) void k_sum_i64(const int64_t *__restrict__ v, int64_t n,
+                                                 unsigned long long *__restrict__ out) {
+    __shared__ int64_t sh[4];
+    int64_t acc = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        acc += v[i];
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, (unsigned long long)(sh[0] + sh[1] + sh[2] + sh[3]));
 }
 
 // nn and core flags in input order (v_coord_nn, data_process.h:57), and core
@@ -650,49 +680,49 @@ static void grid_classify_d(Ctx &c) {
     const int rows = c.rows_per_cell = D == 3 ? 25 : D == 2 ? 5 : 1;
     c.ranges.ensure((size_t)nc * rows * 8);
     c.cell_c.ensure((size_t)nc * 3 * 4);
-    c.tile_off.ensure((nc + 1) * 4);
-    c.cid.ensure(nc * 4);
-    int32_t *work = c.cid.as<int32_t>();  // reused: label-pass tiles per cell
     {
         Launch L(c.prof, s, "stencil");
-        k_stencil<D><<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), c.cell_start.as<int32_t>(), nc, ig, c.lin_bits,
+        k_stencil<D><<<(unsigned)ceil_div(nc * rows, 256), 256, 0, s>>>(c.cell_key.as<uint64_t>(), c.cell_start.as<int32_t>(), nc, ig, c.lin_bits,
                                          c.row_keys.as<uint64_t>(), c.row_vals.as<int2>(), cap - 1,
-                                         c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), work);
+                                         c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>());
     }
-    scan_exclusive_i32(work, c.tile_off.as<int32_t>(), nc, d_total, c.scratch, s, c.prof);
     // count tiles over groups of cells of one row
     c.gflag.ensure(nc * 4);
     c.ctile_off.ensure((nc + 1) * 4);
     {
         Launch L(c.prof, s, "group_tiles");
-        k_group_flags<<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), c.cell_c.as<int32_t>(), nc, c.gflag.as<int32_t>());
+        k_group_flags<<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), c.cell_c.as<int32_t>(), nc, c.group_cells,
+                                          c.gflag.as<int32_t>());
         k_group_tiles<<<ncb, 256, 0, s>>>(c.gflag.as<int32_t>(), c.cell_start.as<int32_t>(), nc, c.flag.as<int32_t>());
     }
-    scan_exclusive_i32(c.flag.as<int32_t>(), c.ctile_off.as<int32_t>(), nc, d_total + 1, c.scratch, s, c.prof);
-    int32_t h_tot2[2] = {0, 0};
-    NDBX_HIP(hipMemcpyAsync(h_tot2, d_total, 8, hipMemcpyDeviceToHost, s));
+    scan_exclusive_i32(c.flag.as<int32_t>(), c.ctile_off.as<int32_t>(), nc, d_total, c.scratch, s, c.prof);
+    NDBX_HIP(hipMemcpyAsync(&h_total, d_total, 4, hipMemcpyDeviceToHost, s));
     c.sync();
-    const int64_t nt = h_tot2[0], nct = h_tot2[1];
+    const int64_t nct = h_total;
     c.n_tiles = nct;
-    c.n_label_tiles = nt;
-    c.tiles.ensure(nt * 8);
     c.ctiles.ensure(nct * 16);
     {
         Launch L(c.prof, s, "tiles");
-        k_tiles<<<ncb, 256, 0, s>>>(c.cell_start.as<int32_t>(), c.tile_off.as<int32_t>(), nc, c.tiles.as<int2>());
         k_group_tile_fill<<<ncb, 256, 0, s>>>(c.gflag.as<int32_t>(), c.cell_start.as<int32_t>(),
                                               c.cell_of.as<int32_t>(), c.ctile_off.as<int32_t>(), nc,
                                               c.ctiles.as<int4>());
     }
     c.nn_s.ensure(n * 4);
     const bool stats = c.prof.on;
-    if (stats) fill_u64(reinterpret_cast<uint64_t *>(d_tests), 0, 1, s, c.prof);
+    if (stats) {
+        c.tile_tests.ensure(nct * 8);
+        fill_u64(reinterpret_cast<uint64_t *>(d_tests), 0, 1, s, c.prof);
+    }
     {
         Launch L(c.prof, s, "count");
         k_count<D><<<(unsigned)ceil_div(nct, 4), 256, 0, s>>>(
             c.ctiles.as<int4>(), nct, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig, c.lin_bits,
             c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), nc, c.soa.as<float>(), npad, c.nn_s.as<int32_t>(),
-            stats ? d_tests : nullptr);
+            stats ? c.tile_tests.as<int64_t>() : nullptr);
+    }
+    if (stats) {
+        Launch L(c.prof, s, "stats");
+        k_sum_i64<<<1024, 256, 0, s>>>(c.tile_tests.as<int64_t>(), nct, d_tests);
     }
     c.core_s.ensure(n);
     {

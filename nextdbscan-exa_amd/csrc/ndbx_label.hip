@@ -13,6 +13,7 @@
 #include <climits>
 
 #include "ndbx_internal.h"
+#include "ndbx_tile.h"
 
 namespace ndbx {
 
@@ -96,15 +97,6 @@ __global__ __launch_bounds__(256) void k_soa_core(const float *__restrict__ soa,
     for (int k = 0; k < D; ++k) soa_core[k * npad + i] = c ? soa[k * npad + i] : __builtin_nanf("");
 }
 
-__device__ __forceinline__ float wave_min(float v) {
-    for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
-__device__ __forceinline__ float wave_max(float v) {
-    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
-
 // Any of U candidates [j, j+U) within eps of this lane's query.
 template <int D, int U>
 __device__ __forceinline__ bool any_block(const float (&qx)[D], const float *__restrict__ soa, int64_t npad, int j,
@@ -153,7 +145,8 @@ __device__ __forceinline__ bool wave_any_within(const float (&qx)[D], const floa
 constexpr int kMaxStencilCells = 128;  // 25 rows x 5 cells for d <= 3
 
 template <int D>
-__global__ __launch_bounds__(256) void k_union(int64_t nc, int phase, const int32_t *__restrict__ cell_start,
+__global__ __launch_bounds__(256) void k_union(const int32_t *__restrict__ list, const int32_t *__restrict__ n_list,
+                                               int64_t nc, int phase, const int32_t *__restrict__ cell_start,
                                                const uint64_t *__restrict__ cell_key,
                                                const InstGrid *__restrict__ ig, int lin_bits,
                                                const int2 *__restrict__ ranges, int rows_per_cell,
@@ -164,10 +157,9 @@ __global__ __launch_bounds__(256) void k_union(int64_t nc, int phase, const int3
     __shared__ int32_t cand[4][kMaxStencilCells];
     __shared__ int32_t cand_root[4][kMaxStencilCells];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t A64 = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
-    if (A64 >= nc) return;
-    const int A = (int)A64;
-    if (!cell_has_core[A]) return;
+    const int64_t li = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+    if (li >= *n_list) return;
+    const int A = list[li];
     const int lane = threadIdx.x & 63;
     const InstGrid &g = ig[cell_key[A] >> lin_bits];
     const float e2 = g.e2;
@@ -292,6 +284,97 @@ __global__ __launch_bounds__(256) void k_union(int64_t nc, int phase, const int3
     }
 }
 
+// Core cells split by size: cells with more than kSmallCell points go to the
+// wave-per-cell union, the rest to the lane-per-cell union.
+constexpr int kSmallCell = 8;
+
+__global__ __launch_bounds__(256) void k_cell_lists(const int32_t *__restrict__ cell_has_core,
+                                                    const int32_t *__restrict__ cell_start, int64_t nc,
+                                                    int32_t *__restrict__ big, int32_t *__restrict__ small,
+                                                    int32_t *__restrict__ counts) {
+    __shared__ int32_t wb[4], ws[4], bb, bs;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool v = c < nc && cell_has_core[c];
+    const bool is_big = v && cell_start[c + 1] - cell_start[c] > kSmallCell;
+    const bool is_small = v && !is_big;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t mb = __ballot(is_big), ms = __ballot(is_small);
+    if (lane == 0) {
+        wb[w] = __popcll(mb);
+        ws[w] = __popcll(ms);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // one atomic per block and list
+        const int tb = wb[0] + wb[1] + wb[2] + wb[3], ts = ws[0] + ws[1] + ws[2] + ws[3];
+        bb = tb ? atomicAdd(counts + 0, tb) : 0;
+        bs = ts ? atomicAdd(counts + 1, ts) : 0;
+    }
+    __syncthreads();
+    int ob = bb, os = bs;
+    for (int i = 0; i < w; ++i) {
+        ob += wb[i];
+        os += ws[i];
+    }
+    if (is_big) big[ob + __popcll(mb & lt)] = (int32_t)c;
+    if (is_small) small[os + __popcll(ms & lt)] = (int32_t)c;
+}
+
+// One lane per small core cell A: every stencil neighbour B > A with a core
+// and a different root is tested pair by pair (its cores against A's cores,
+// non-cores are NaN in soa_core) until a pair within eps unites them.
+template <int D>
+__global__ __launch_bounds__(256) void k_union_small(const int32_t *__restrict__ list,
+                                                     const int32_t *__restrict__ n_list,
+                                                     const int32_t *__restrict__ cell_start,
+                                                     const uint64_t *__restrict__ cell_key,
+                                                     const InstGrid *__restrict__ ig, int lin_bits,
+                                                     const int2 *__restrict__ ranges, int rows_per_cell,
+                                                     const float *__restrict__ soa_core, int64_t npad,
+                                                     const int32_t *__restrict__ cell_has_core,
+                                                     int32_t *__restrict__ parent) {
+    const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= *n_list) return;
+    const int A = list[li];
+    const float e2 = ig[cell_key[A] >> lin_bits].e2;
+    const int a0 = cell_start[A], a1 = cell_start[A + 1];
+    float ax[kSmallCell][D];
+    int na = 0;
+    for (int i = a0; i < a1; ++i) {
+        float p[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) p[k] = soa_core[k * npad + i];
+        if (p[0] != p[0]) continue;  // not a core
+#pragma unroll
+        for (int k = 0; k < D; ++k) ax[na][k] = p[k];
+        ++na;
+    }
+    for (int slot = 0; slot < rows_per_cell; ++slot) {
+        const int2 w = ranges[(int64_t)A * rows_per_cell + slot];
+        for (int B = w.x > A ? w.x : A + 1; B < w.y; ++B) {
+            if (!cell_has_core[B]) continue;
+            if (uf_find(parent, A) == uf_find(parent, B)) continue;
+            const int b1 = cell_start[B + 1];
+            bool hit = false;
+            for (int j = cell_start[B]; j < b1 && !hit; ++j) {
+                float p[D];
+#pragma unroll
+                for (int k = 0; k < D; ++k) p[k] = soa_core[k * npad + j];
+                for (int i = 0; i < na && !hit; ++i) {
+                    float len = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < D; ++k) {
+                        const float df = ax[i][k] - p[k];
+                        len = __fadd_rn(len, __fmul_rn(df, df));
+                    }
+                    hit = len <= e2;
+                }
+            }
+            if (hit) uf_unite(parent, A, B);
+        }
+    }
+}
+
 // Every core cell's root and the lowest core id of each root's set.
 __global__ __launch_bounds__(256) void k_flatten(const int32_t *__restrict__ cell_has_core,
                                                  const int32_t *__restrict__ cell_core_min, int64_t nc,
@@ -332,43 +415,55 @@ __global__ __launch_bounds__(256) void k_cell_label(const int32_t *__restrict__ 
 // Labels per point: cores take their cell's id; a non-core point takes the
 // smallest id among core cells holding a core within eps (its own cell's
 // cores are within eps by construction).
+// One wave per count tile.  Cores keep their cell's id.  A non-core point
+// starts from its own cell's id (the cores of its cell are within eps by
+// construction) and takes the id of any core cell B with a smaller id that
+// holds a core within eps.  Windows are the count windows of the tile, over
+// the box of its non-core points.
 template <int D>
-__global__ __launch_bounds__(256) void k_border(const int2 *__restrict__ tiles, int64_t n_tiles,
+__global__ __launch_bounds__(256) void k_border(const int4 *__restrict__ ctiles, int64_t n_tiles,
                                                 const int32_t *__restrict__ cell_start,
                                                 const uint64_t *__restrict__ cell_key,
                                                 const InstGrid *__restrict__ ig, int lin_bits,
-                                                const int2 *__restrict__ ranges,
-                                                int rows_per_cell, const float *__restrict__ soa,
-                                                const float *__restrict__ soa_core, int64_t npad,
-                                                const uint8_t *__restrict__ core_s,
+                                                const int2 *__restrict__ ranges, int rows_per_cell,
+                                                const int32_t *__restrict__ cell_c, int64_t nc,
+                                                const int32_t *__restrict__ cell_of,
+                                                const float *__restrict__ soa, const float *__restrict__ soa_core,
+                                                int64_t npad, const uint8_t *__restrict__ core_s,
                                                 const int32_t *__restrict__ cell_label,
                                                 const uint32_t *__restrict__ perm, int32_t *__restrict__ labels) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
     if (t >= n_tiles) return;
     const int lane = threadIdx.x & 63;
-    const int2 tile = tiles[t];
-    const int A = tile.x;
-    const int q = tile.y + lane;
-    const bool valid = q < cell_start[A + 1];
+    const int4 tile = ctiles[t];
+    const bool valid = lane < tile.w;
+    const int q = tile.z + (valid ? lane : 0);
     const bool is_core = valid && core_s[q];
-    const int32_t LA = cell_label[A];
-    int32_t best = LA;  // NO_CLUSTER when A has no core
+    int32_t best = cell_label[cell_of[q]];  // NO_CLUSTER when the cell has no core
     const bool need = valid && !is_core;
     if (__ballot(need) != 0) {
-        const float e2 = ig[cell_key[A] >> lin_bits].e2;
-        float qx[D];
+        const InstGrid &g = ig[cell_key[tile.x] >> lin_bits];
+        const float e2 = g.e2;
+        float qx[D], lo[D], hi[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) qx[k] = soa[k * npad + (valid ? q : tile.y)];
-        const int nr = rows_per_cell;
-        const int2 *rr = ranges + (int64_t)A * rows_per_cell;
-        for (int r = 0; r < nr; ++r) {
-            const int2 cr = rr[r];
-            for (int B = cr.x; B < cr.y; ++B) {
+        for (int k = 0; k < D; ++k) {
+            qx[k] = soa[k * npad + q];
+            lo[k] = wave_min(need ? qx[k] : __builtin_inff());
+            hi[k] = wave_max(need ? qx[k] : -__builtin_inff());
+        }
+        const int2 w = tile_window<D>(lane, rows_per_cell, tile.x, tile.y, lo, hi, g, cell_c[nc + tile.x],
+                                      cell_c[2 * nc + tile.x], ranges, cell_c, nc);
+        uint64_t rows = __ballot(w.x < w.y);
+        while (rows) {
+            const int slot = __builtin_ctzll(rows);
+            rows &= rows - 1;
+            const int b_end = __builtin_amdgcn_readlane(w.y, slot);
+            for (int B = __builtin_amdgcn_readlane(w.x, slot); B < b_end; ++B) {
                 const int32_t LB = cell_label[B];
                 // only lanes that would improve take part; others see NaN
                 const bool want = need && LB < best;
-                if (B == A || LB == NDBX_NO_CLUSTER || __ballot(want) == 0) continue;
+                if (LB == NDBX_NO_CLUSTER || __ballot(want) == 0) continue;
                 float qw[D];
 #pragma unroll
                 for (int k = 0; k < D; ++k) qw[k] = want ? qx[k] : __builtin_nanf("");
@@ -466,7 +561,7 @@ __global__ __launch_bounds__(256) void k_meta_cells(const int32_t *__restrict__ 
 
 template <int D>
 static void grid_label_d(Ctx &c) {
-    const int64_t n = c.n, nc = c.n_cells, nt = c.n_label_tiles;
+    const int64_t n = c.n, nc = c.n_cells;
     const hipStream_t s = c.stream;
     const int nb = (int)ceil_div(n, 256), ncb = (int)ceil_div(nc, 256);
     InstGrid *ig = c.inst_d.as<InstGrid>();
@@ -498,19 +593,35 @@ static void grid_label_d(Ctx &c) {
         k_iota<<<ncb, 256, 0, s>>>(c.parent.as<int32_t>(), nc);
     }
     fill_i32(c.comp_min.as<int32_t>(), INT_MAX, nc, s, c.prof);
-    const unsigned tb = (unsigned)ceil_div(nt, 4);
     c.soa_core.ensure((size_t)npad * D * 4);
     {
         Launch L(c.prof, s, "soa_core");
         k_soa_core<D><<<nb, 256, 0, s>>>(c.soa.as<float>(), c.core_s.as<uint8_t>(), n, npad, c.soa_core.as<float>());
     }
+    // compacted core-cell lists (big: wave per cell; small: lane per cell)
+    c.gflag.ensure(nc * 4);
+    c.ctile_off.ensure(nc * 4);
+    int32_t *big = c.gflag.as<int32_t>(), *small = c.ctile_off.as<int32_t>();
+    int32_t *counts = c.misc.as<int32_t>() + 8;
+    fill_i32(counts, 0, 2, s, c.prof);
+    {
+        Launch L(c.prof, s, "cell_lists");
+        k_cell_lists<<<ncb, 256, 0, s>>>(c.cell_has_core.as<int32_t>(), c.cell_start.as<int32_t>(), nc, big, small,
+                                         counts);
+    }
     for (int phase = 0; phase < 2; ++phase) {
         Launch L(c.prof, s, phase == 0 ? "union_faces" : "union");
-        k_union<D><<<(unsigned)ceil_div(nc, 4), 256, 0, s>>>(nc, phase, c.cell_start.as<int32_t>(),
+        k_union<D><<<(unsigned)ceil_div(nc, 4), 256, 0, s>>>(big, counts, nc, phase, c.cell_start.as<int32_t>(),
                                                              c.cell_key.as<uint64_t>(), ig, c.lin_bits,
                                                              c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(),
                                                              c.soa_core.as<float>(), npad,
                                                              c.cell_has_core.as<int32_t>(), c.parent.as<int32_t>());
+    }
+    {
+        Launch L(c.prof, s, "union_small");
+        k_union_small<D><<<ncb, 256, 0, s>>>(small, counts + 1, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(),
+                                             ig, c.lin_bits, c.ranges.as<int2>(), rows, c.soa_core.as<float>(), npad,
+                                             c.cell_has_core.as<int32_t>(), c.parent.as<int32_t>());
     }
     {
         Launch L(c.prof, s, "flatten");
@@ -525,10 +636,12 @@ static void grid_label_d(Ctx &c) {
     }
     {
         Launch L(c.prof, s, "border");
-        k_border<D><<<tb, 256, 0, s>>>(c.tiles.as<int2>(), nt, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(),
-                                       ig, c.lin_bits, c.ranges.as<int2>(), rows,
-                                       c.soa.as<float>(), c.soa_core.as<float>(), npad, c.core_s.as<uint8_t>(),
-                                       c.cell_label.as<int32_t>(), c.vals.as<uint32_t>(), c.labels.as<int32_t>());
+        const int64_t nct = c.n_tiles;
+        k_border<D><<<(unsigned)ceil_div(nct, 4), 256, 0, s>>>(
+            c.ctiles.as<int4>(), nct, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig, c.lin_bits,
+            c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), nc, c.cell_of.as<int32_t>(), c.soa.as<float>(),
+            c.soa_core.as<float>(), npad, c.core_s.as<uint8_t>(), c.cell_label.as<int32_t>(), c.vals.as<uint32_t>(),
+            c.labels.as<int32_t>());
     }
     fill_i32(c.meta.as<int32_t>(), 0, 3 * c.n_inst, s, c.prof);
     {

@@ -1,0 +1,61 @@
+// Device helpers shared by the tile kernels (count, border): wave reductions
+// and the per-row candidate windows of a tile.
+#pragma once
+
+#include "ndbx_internal.h"
+
+namespace ndbx {
+
+__device__ __forceinline__ float wave_min(float v) {
+    for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// Search radius^2 above anything the float32 predicate can accept: e2 (1 + 2^-18).
+__device__ __forceinline__ double search_r2(float e2) { return (double)e2 * (1.0 + 3.814697265625e-06); }
+
+// Candidate cells of one stencil row of a tile, computed by lane `slot`
+// (slot < rows_per_cell; other lanes return an empty window).  The tile is a
+// run of cells [cf, cl] of one row; its bounding box is [lo, hi].  Rows
+// whose perpendicular gap to the box exceeds the search radius are empty;
+// along dimension 0 only cells that can hold a point within the radius of the
+// box are kept.  Cell bounds are widened by 2^-20 of a cell, so no point the
+// predicate accepts is dropped.  Returns the cell index range [a, e).
+template <int D>
+__device__ __forceinline__ int2 tile_window(int slot, int rows_per_cell, int cf, int cl, const float (&lo)[D],
+                                            const float (&hi)[D], const InstGrid &g, int c1, int c2,
+                                            const int2 *__restrict__ ranges, const int32_t *__restrict__ cell_c,
+                                            int64_t nc) {
+    if (slot >= rows_per_cell) return make_int2(0, 0);
+    int a = ranges[(int64_t)cf * rows_per_cell + slot].x;
+    int e = ranges[(int64_t)cl * rows_per_cell + slot].y;
+    if (a >= e) return make_int2(0, 0);
+    const double inv_s = g.inv_s, s = 1.0 / inv_s, slack = s * 9.5367431640625e-07;  // 2^-20 cell
+    double rem = search_r2(g.e2);
+    if (D >= 2) {
+        const int dy = (D == 3 ? slot % 5 : slot) - 2;
+        const double ylo = g.origin[1] + (double)(c1 + dy) * s - slack, yhi = ylo + s + 2.0 * slack;
+        const double gy = fmax(0.0, fmax(ylo - (double)hi[1], (double)lo[1] - yhi));
+        rem -= gy * gy;
+    }
+    if (D >= 3) {
+        const int dz = slot / 5 - 2;
+        const double zlo = g.origin[2] + (double)(c2 + dz) * s - slack, zhi = zlo + s + 2.0 * slack;
+        const double gz = fmax(0.0, fmax(zlo - (double)hi[2], (double)lo[2] - zhi));
+        rem -= gz * gz;
+    }
+    if (rem < 0.0) return make_int2(0, 0);
+    const double rx = sqrt(rem) + slack;
+    const int xl = (int)floor(((double)lo[0] - rx - g.origin[0]) * inv_s);
+    const int xh = (int)floor(((double)hi[0] + rx - g.origin[0]) * inv_s);
+    while (a < e && cell_c[a] < xl) ++a;
+    while (e > a && cell_c[e - 1] > xh) --e;
+    return make_int2(a, e);
+}
+
+}  // namespace ndbx
