@@ -39,6 +39,7 @@ def algorithmic_bytes(kernel, n, d, cells):
     its outputs), per kernel: N points of d float32, C_ne non-empty cells."""
     table = {
         "count": n * (4 * d + 4) + 8 * cells,          # sorted coords in, nn out, cell table
+        "mf_count": n * (4 * d + 4),                    # coordinates in, nn out
         "union": n * (4 * d + 1) + 8 * cells,          # coords + core flags in, parents
         "border": n * (4 * d + 1 + 4) + 8 * cells,     # coords + core flags in, labels out
         "radix_scatter": n * 24,                        # key+val in and out
@@ -107,6 +108,7 @@ def main():
     ap.add_argument("--n", type=int, default=None, help="smaller instance of the same workload shape")
     ap.add_argument("--n-inst", type=int, default=4096, help="instances of the batched workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU form comparison")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one device-to-device copy of BYTES (a known-size kernel for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -173,6 +175,23 @@ def main():
                                                                                              repr(float(eps)))
         ctx = ndbx.Context(x, eps, m, flags=ndbx.PROFILE)
         step = lambda: (ctx.classify().label(), ctx.meta())
+        if d > 3 and not args.no_plain:
+            # plain GPU form for comparison: the all-pairs float32 kernels
+            with ndbx.Context(x, eps, m, flags=ndbx.FORCE_BRUTE) as pc:
+                pc.classify().label()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                pc.classify().label()
+                pmeta = pc.meta()
+                plain_ms = (time.perf_counter() - t0) * 1e3
+                plain_nn = pc.nn()
+            extra["plain_gpu"] = {"ms_per_step": plain_ms, "value": n / (plain_ms * 1e-3),
+                                  "form": "all-pairs float32 kernels (FORCE_BRUTE), 1 timed step"}
+        if d > 3:
+            with ndbx.Context(x, eps, m, flags=ndbx.FP64_AUDIT) as ac:
+                ac.classify()
+                extra["fp64_audit"] = ac.audit()
+                extra["fp64_audit"]["scope"] = "every pair of ~2048 sampled 32-row tiles"
 
     for _ in range(args.warmup):
         step()
@@ -193,6 +212,8 @@ def main():
     if isinstance(meta, np.ndarray):
         meta = [int(v) for v in meta.sum(0)]
     ctx.close()
+    if "plain_gpu" in extra and not args.workload.startswith("batch3d"):
+        extra["plain_gpu"]["same_result"] = bool(tuple(pmeta) == tuple(meta) and np.array_equal(plain_nn, nn))
 
     kernels = {k: {"ms_avg": v[0] / max(v[1], 1), "launches_per_step": v[1] / args.steps}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}

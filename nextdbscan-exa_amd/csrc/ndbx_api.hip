@@ -82,6 +82,7 @@ static void run_classify(Ctx &c) {
         return;
     }
     if (c.path == PATH_GRID) grid_classify(c);
+    else if (c.path == PATH_MFMA) mfma_classify(c);
     else brute_classify(c);
     c.classified = true;
 }
@@ -92,6 +93,8 @@ static void run_label(Ctx &c) {
         fill_i32(c.meta.as<int32_t>(), 0, 3 * c.n_inst, c.stream, c.prof);
     } else if (c.path == PATH_GRID) {
         grid_label(c);
+    } else if (c.path == PATH_MFMA) {
+        mfma_label(c);
     } else {
         brute_label(c);
     }

@@ -86,8 +86,6 @@ static int bits_for(uint64_t v) {  // bits needed to hold values 0..v-1
 // Decides the path and, for the grid, each instance's cell geometry.
 void grid_plan(Ctx &c) {
     c.path = PATH_GRID;
-    if (c.flags & NDBX_FORCE_BRUTE) c.path = PATH_BRUTE;
-    if (c.d > 3) c.path = PATH_BRUTE;
     if (c.n == 0) {
         c.h_inst.clear();
         return;
@@ -173,6 +171,17 @@ void grid_plan(Ctx &c) {
         }
         lin_bits = std::max(lin_bits, bits_for(total));
     }
+    // d > 3: the MFMA filter where it applies (finite input, normal e2,
+    // d <= 128, one instance), else all pairs
+    if (c.d > 3) {
+        bool ok = c.n_inst == 1 && mfma_supported(c.d);
+        for (int i = 0; i < c.n_inst && ok; ++i) {
+            const float e2 = c.h_eps[i] * c.h_eps[i];
+            ok = special[i] == 0 && std::isfinite(e2) && e2 >= 1e-30f;
+        }
+        c.path = ok ? PATH_MFMA : PATH_BRUTE;
+    }
+    if (c.flags & NDBX_FORCE_BRUTE) c.path = PATH_BRUTE;
     const int inst_bits = bits_for((uint64_t)c.n_inst);
     c.lin_bits = lin_bits;
     c.cell_bits = lin_bits + inst_bits;
@@ -579,6 +588,21 @@ __global__ __launch_bounds__(256) void k_sum_i64(const int64_t *__restrict__ v, 
     if (threadIdx.x == 0) atomicAdd(out, (unsigned long long)(sh[0] + sh[1] + sh[2] + sh[3]));
 }
 
+// Cell table from sorted keys: flags, exclusive scan, cell of each point,
+// first point and key of each cell; *total (device) = number of cells.
+void cell_table(const uint64_t *keys, int64_t n, int sub_bits, int32_t *flag, int32_t *excl, int32_t *cell_of,
+                int32_t *cell_start, uint64_t *cell_key, int32_t *total, DevBuf &scratch, hipStream_t s,
+                Profiler &prof) {
+    const int nb = (int)ceil_div(n, 256);
+    {
+        Launch L(prof, s, "cell_flags");
+        k_cell_flags<<<nb, 256, 0, s>>>(keys, n, sub_bits, flag);
+    }
+    scan_exclusive_i32(flag, excl, n, total, scratch, s, prof);
+    Launch L(prof, s, "cell_fill");
+    k_cell_fill<<<nb, 256, 0, s>>>(keys, sub_bits, flag, excl, n, cell_of, cell_start, cell_key);
+}
+
 // nn and core flags in input order (v_coord_nn, data_process.h:57), and core
 // flags in sorted order for the label pass.
 __global__ __launch_bounds__(256) void k_classify_out(const int32_t *__restrict__ nn_s,
@@ -643,16 +667,8 @@ static void grid_classify_d(Ctx &c) {
     c.misc.ensure(64);
     int32_t *d_total = c.misc.as<int32_t>();
     unsigned long long *d_tests = reinterpret_cast<unsigned long long *>(c.misc.as<int32_t>() + 4);
-    {
-        Launch L(c.prof, s, "cell_flags");
-        k_cell_flags<<<nb, 256, 0, s>>>(keys, n, c.sub_bits, c.flag.as<int32_t>());
-    }
-    scan_exclusive_i32(c.flag.as<int32_t>(), c.cid.as<int32_t>(), n, d_total, c.scratch, s, c.prof);
-    {
-        Launch L(c.prof, s, "cell_fill");
-        k_cell_fill<<<nb, 256, 0, s>>>(keys, c.sub_bits, c.flag.as<int32_t>(), c.cid.as<int32_t>(), n,
-                                       c.cell_of.as<int32_t>(), c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>());
-    }
+    cell_table(keys, n, c.sub_bits, c.flag.as<int32_t>(), c.cid.as<int32_t>(), c.cell_of.as<int32_t>(),
+               c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), d_total, c.scratch, s, c.prof);
     const int64_t npad = ceil_div(n, 64) * 64;
     c.soa.ensure((size_t)npad * D * 4);
     {

@@ -163,7 +163,12 @@ void grid_classify(Ctx &c);        // keys, sort, cells, stencil, counts
 void grid_label(Ctx &c);           // union-find over cells, border, labels
 void brute_classify(Ctx &c);
 void brute_label(Ctx &c);
-void finish_meta(Ctx &c);          // meta reduction (both paths)
+void mfma_classify(Ctx &c);        // d >= 4: partition, bf16 MFMA filter, counts
+void mfma_label(Ctx &c);
+bool mfma_supported(int d);
+void cell_table(const uint64_t *keys, int64_t n, int sub_bits, int32_t *flag, int32_t *excl, int32_t *cell_of,
+                int32_t *cell_start, uint64_t *cell_key, int32_t *total, DevBuf &scratch, hipStream_t s,
+                Profiler &prof);
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 

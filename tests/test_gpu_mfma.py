@@ -1,0 +1,63 @@
+"""The MFMA distance filter (4 <= d <= 128): every neighbour decision equals
+the reference's float32 predicate, so nn, core flags, labels and meta are
+bit-exact against the oracle; the float64 audit must report zero decisions
+of the bf16 estimate outside its guard band that the float32 predicate
+contradicts."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+import synth
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # (d, n, n_blobs, extent, sigma range, noise, eps, m, seed)
+    (4, 6000, 8, 40.0, (0.5, 2.0), 0.05, 2.5, 8, 1),
+    (8, 8000, 10, 40.0, (0.5, 1.5), 0.05, 3.0, 10, 2),
+    (16, 8000, 12, 40.0, (0.5, 1.5), 0.1, 5.0, 10, 3),
+    (33, 5000, 6, 30.0, (0.5, 1.0), 0.1, 6.0, 6, 4),
+    (64, 6000, 10, 30.0, (0.7, 1.2), 0.1, 10.5, 15, 5),
+    (100, 3000, 5, 30.0, (0.5, 1.0), 0.1, 11.0, 5, 6),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"d{c[0]}_n{c[1]}")
+def test_mfma_against_oracle(gpu_lib, case):
+    d, n, nb, ext, sr, noise, eps, m, seed = case
+    x = synth.blobs(n, d, nb, 0.0, ext, None, noise, 900 + seed, sigma_range=sr)
+    eps = np.float32(eps)
+    print(f"case mfma d={d} n={n}", flush=True)
+    with gpu_lib.Context(x, eps, m, flags=gpu_lib.FP64_AUDIT) as c:
+        c.classify().label()
+        nn, lab, meta, path, audit = c.nn(), c.labels(), c.meta(), c.path(), c.audit()
+    nn_o, lab_o, meta_o = O.dbscan(x, eps, m)
+    assert path == "mfma-filter"
+    assert audit["filter_flips"] == 0, audit
+    np.testing.assert_array_equal(nn, nn_o)
+    np.testing.assert_array_equal(lab, lab_o)
+    assert meta == meta_o
+
+
+def test_mfma_clusters64d(gpu_lib):
+    x, eps = synth.clusters64d_1m(20000, seed=1003)
+    print("case mfma clusters64d n=20000", flush=True)
+    with gpu_lib.Context(x, eps, 15, flags=gpu_lib.FP64_AUDIT) as c:
+        c.classify().label()
+        nn, lab, meta, audit = c.nn(), c.labels(), c.meta(), c.audit()
+    nn_o, lab_o, meta_o = O.dbscan(x, eps, 15)
+    assert audit["filter_flips"] == 0, audit
+    assert audit["max_abs_err"] < 0.01 * float(eps) ** 2, audit
+    np.testing.assert_array_equal(nn, nn_o)
+    np.testing.assert_array_equal(lab, lab_o)
+    assert meta == meta_o
+
+
+def test_mfma_matches_all_pairs(gpu_lib):
+    x = synth.blobs(7000, 24, 9, 0.0, 30.0, None, 0.1, 31, sigma_range=(0.5, 1.5))
+    eps = np.float32(6.0)
+    a = gpu_lib.dbscan(x, eps, 9)
+    b = gpu_lib.dbscan(x, eps, 9, flags=gpu_lib.FORCE_BRUTE)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    assert a[2] == b[2]
