@@ -14,7 +14,7 @@
 // 3. Guard band: |est - len| <= band, len being the reference's float32
 //    direct-difference sum (data_process.cpp:443-447), with
 //       band = 2 * ( B1 * R^2 + 2^-21 * sqrt(e2) * R + (d+3) 2^-24 e2 ),
-//       B1 = 2 (3.1 2^-16 + 3 Kp 2^-24) + (2 Kp + 8) 2^-24,
+//       B1 = 2 (3.1 2^-16 + 3 Kp 2^-24) + (2 Kp + 16) 2^-24,
 //    R = largest |a'| of the tile's rows and the candidate cell.  Pairs with
 //    est <= e2 - band are neighbours, est > e2 + band are not, and the rest
 //    are decided by the reference's own float32 arithmetic on the original
@@ -241,6 +241,9 @@ __device__ __forceinline__ bool exact_pred(const float *__restrict__ p, const fl
 struct MfmaArgs {
     const int4 *tiles;
     int64_t n_tiles;
+    const int4 *groups;
+    int64_t n_groups;
+    const int32_t *rows;  // tile row -> point (null: identity)
     const int32_t *cand_off, *cand, *cell_start;
     const __bf16 *hi, *lo;
     const float *na, *rho, *cell_rmax, *xs;
@@ -258,25 +261,56 @@ struct MfmaArgs {
     int audit_stride;
 };
 
+// Work groups: up to 4 consecutive row tiles of one cell, one workgroup (4
+// waves, one tile each).  The waves share every 32-column B block through LDS.
+__global__ __launch_bounds__(256) void k_groups(const int32_t *__restrict__ toff, int64_t nc,
+                                                const int32_t *__restrict__ goff, int4 *__restrict__ groups) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int32_t t0 = toff[c], t1 = toff[c + 1];
+    int32_t g = goff[c];
+    for (int32_t t = t0; t < t1; t += 4) groups[g++] = make_int4((int32_t)c, t, t1 - t < 4 ? t1 - t : 4, 0);
+}
+
+__global__ __launch_bounds__(256) void k_group_count(const int32_t *__restrict__ toff, int64_t nc,
+                                                     int32_t *__restrict__ cnt) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nc) cnt[c] = (toff[c + 1] - toff[c] + 3) / 4;
+}
+
 // PASS 0: neighbour counts.  PASS 1: unite core pairs (col > row).  PASS 2:
 // smallest core cluster id within eps for non-core rows.  PASS 3: audit of
-// every estimate of every audit_stride-th tile against float32 and float64.
+// every estimate of every audit_stride-th group against float32 and float64.
+// Rows and columns that take no part are masked by an infinite norm, so the
+// estimate never passes; the few estimates inside the guard band are decided
+// afterwards with the exact predicate, under a ballot.
 template <int KP, int PASS>
 __global__ __launch_bounds__(256) void k_mfma_pass(MfmaArgs A) {
+    constexpr int KS = KP / 16;
+    constexpr int UNITS = 32 * (KP / 8);  // bf16x8 units of one 32-column block, per hi / lo
+    constexpr int ROWU = KP / 8 + 1;      // LDS row stride in units: one pad unit breaks bank conflicts
+    __shared__ bf16x8_t sb_hi[2][32 * ROWU], sb_lo[2][32 * ROWU];  // double buffer
+    __shared__ float sb_na[2][32];
+    __shared__ int32_t sb_aux[2][32];  // PASS 1: column root; PASS 2: column cluster id
+    __shared__ int32_t any_work;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
-    if (PASS == 3) t *= A.audit_stride;
-    if (t >= A.n_tiles) return;
+    int64_t gidx = blockIdx.x;
+    if (PASS == 3) gidx *= A.audit_stride;
+    if (gidx >= A.n_groups) return;
+    const int4 grp = A.groups[gidx];
+    const bool wave_on = wave < grp.z;
+    const int4 tile = A.tiles[grp.y + (wave_on ? wave : 0)];
     const int lane = threadIdx.x & 63;
     const int half = lane >> 5, col = lane & 31;
-    const int4 tile = A.tiles[t];
-    const int cell = tile.x, row0 = tile.y, nrows = tile.z;
-    constexpr int KS = KP / 16;
+    const int cell = grp.x, row0 = tile.y, nrows = wave_on ? tile.z : 0;
     const bf16x8_t *hi8 = reinterpret_cast<const bf16x8_t *>(A.hi);
     const bf16x8_t *lo8 = reinterpret_cast<const bf16x8_t *>(A.lo);
+    const float INF = __builtin_inff();
+    // tile row t is point rows[row0 + t] (or row0 + t)
+    auto row_point = [&](int t) -> int64_t { return A.rows ? (int64_t)A.rows[row0 + t] : (int64_t)(row0 + t); };
     // A operand: row `col` of the tile, k elements [16 ks + 8 half, +8)
     const bool arow_ok = col < nrows;
-    const int64_t arow = row0 + (arow_ok ? col : 0);
+    const int64_t arow = row_point(arow_ok ? col : 0);
     bf16x8_t ahi[KS], alo[KS];
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
@@ -289,98 +323,245 @@ __global__ __launch_bounds__(256) void k_mfma_pass(MfmaArgs A) {
     }
     // rows held in the accumulator: r -> tile row 8 (r/4) + 4 half + (r%4)
     float na_r[16];
-    int32_t acc_i[16];
-    uint32_t rvalid = 0, rcore = 0;
-    int32_t root_r[16];
+    int32_t acc_i[16], rp_r[16], root_r[16];
+    uint32_t rlive = 0;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int tr = 8 * (r >> 2) + 4 * half + (r & 3);
         const bool ok = tr < nrows;
-        const int64_t p = row0 + (ok ? tr : 0);
-        na_r[r] = A.na[p];
-        rvalid |= (uint32_t)ok << r;
+        const int64_t p = row_point(ok ? tr : 0);
+        rp_r[r] = (int32_t)p;
+        bool live = ok;
+        if (PASS == 1) live = ok && A.core_s[p];
+        if (PASS == 2) live = ok && !A.core_s[p];
+        rlive |= (uint32_t)live << r;
+        na_r[r] = live ? A.na[p] : INF;
         acc_i[r] = PASS == 2 ? NDBX_NO_CLUSTER : 0;
-        root_r[r] = 0;
-        if (PASS == 1 || PASS == 2) {
-            const bool c = ok && A.core_s[p];
-            rcore |= (uint32_t)c << r;
-            if (PASS == 1 && c) root_r[r] = mf_find(A.parent, (int32_t)p);
-        }
+        root_r[r] = (PASS == 1 && live) ? mf_find(A.parent, (int32_t)p) : -1;
     }
-    if (PASS == 2 && (__ballot(rvalid & ~rcore) == 0)) return;  // no non-core row
-    if (PASS == 1 && (__ballot(rcore) == 0)) return;
-    float rmax_row = A.rho[arow];
+    if (PASS == 1 || PASS == 2) {  // whole workgroup leaves when no wave has work
+        if (threadIdx.x == 0) any_work = 0;
+        __syncthreads();
+        if (__ballot(rlive != 0) != 0 && lane == 0) any_work = 1;
+        __syncthreads();
+        if (!any_work) return;
+    }
+    float rmax_row = arow_ok ? A.rho[arow] : 0.0f;
     for (int o = 32; o >= 1; o >>= 1) rmax_row = fmaxf(rmax_row, __shfl_xor(rmax_row, o, 64));
     rmax_row = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(rmax_row)));
+    const int32_t min_row_point = A.rows ? 0 : A.tiles[grp.y].y;  // PASS 1 (identity rows only)
 
     float amax = 0.0f;  // audit: largest |est - float64 distance^2|
+    // The column blocks of all candidate cells form one sequence; the B data
+    // of the next block is fetched into registers while the current block is
+    // computed from LDS.
+    constexpr int UPT = (UNITS + 255) / 256;  // units per thread
     const int c_beg = A.cand_off[cell], c_end = A.cand_off[cell + 1];
-    for (int ci = c_beg; ci < c_end; ++ci) {
+    int ci = c_beg, cb = 0, ce = 0, cs = 0;
+    auto load_cell = [&](int ci_, int &cb_, int &cs_, int &ce_) {
+        const int cc_ = A.cand[ci_];
+        cs_ = A.cell_start[cc_];
+        ce_ = A.cell_start[cc_ + 1];
+        cb_ = cs_;
+    };
+    // move (ci, cb) to the first block at or after it that has work; ci ==
+    // c_end when none is left.  PASS 1 jumps over blocks that lie entirely
+    // below every row of the group.
+    auto settle = [&](int &ci_, int &cb_, int &cs_, int &ce_) {
+        while (ci_ < c_end) {
+            if (PASS == 1 && cb_ + 31 < min_row_point) cb_ += 32 * ((min_row_point - cb_) / 32);
+            if (cb_ < ce_) return;
+            ++ci_;
+            if (ci_ < c_end) load_cell(ci_, cb_, cs_, ce_);
+        }
+    };
+    auto advance = [&](int &ci_, int &cb_, int &cs_, int &ce_, bool first) {
+        if (first) {
+            if (ci_ < c_end) load_cell(ci_, cb_, cs_, ce_);
+        } else {
+            cb_ += 32;
+        }
+        settle(ci_, cb_, cs_, ce_);
+    };
+    bf16x8_t pf_hi[UPT], pf_lo[UPT];
+    float pf_na = INF;
+    int32_t pf_aux = 0;
+    auto fetch = [&](int cb_, int cs_, int ce_) {
+#pragma unroll
+        for (int q = 0; q < UPT; ++q) {
+            const int u = threadIdx.x + q * 256;
+            const int bc = u / (KP / 8), kk = u - bc * (KP / 8);
+            const bool ok = u < UNITS && cb_ + bc < ce_;
+            const int64_t src = (int64_t)(ok ? cb_ + bc : cs_) * (KP / 8) + kk;
+            pf_hi[q] = ok ? hi8[src] : bf16x8_t{};
+            pf_lo[q] = ok ? lo8[src] : bf16x8_t{};
+        }
+        if (threadIdx.x < 32) {
+            const int j = cb_ + (int)threadIdx.x;
+            bool live = j < ce_;
+            pf_aux = 0;
+            if (PASS == 1 || PASS == 2) live = live && A.core_s[j];
+            if (PASS == 1 && live) pf_aux = mf_find(A.parent, j);
+            if (PASS == 2 && live) pf_aux = A.label_s[j];
+            pf_na = live ? A.na[j] : INF;
+        }
+    };
+    auto publish = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < UPT; ++q) {
+            const int u = threadIdx.x + q * 256;
+            if (u < UNITS) {
+                const int bc = u / (KP / 8), kk = u - bc * (KP / 8);
+                sb_hi[buf][bc * ROWU + kk] = pf_hi[q];
+                sb_lo[buf][bc * ROWU + kk] = pf_lo[q];
+            }
+        }
+        if (threadIdx.x < 32) {
+            sb_na[buf][threadIdx.x] = pf_na;
+            sb_aux[buf][threadIdx.x] = pf_aux;
+        }
+    };
+    advance(ci, cb, cs, ce, true);
+    if (ci < c_end) {
+        fetch(cb, cs, ce);
+        publish(0);
+    }
+    __syncthreads();
+    int buf = 0;
+    int last_cc = -1;
+    float tin_r[16], tout_r[16];
+    while (ci < c_end) {
         const int cc = A.cand[ci];
-        const double R = fmax((double)rmax_row, (double)A.cell_rmax[cc]);
-        const float band = (float)(2.0 * (A.b1 * R * R + A.b2 * R + A.b3));
-        const float thr_in = A.e2 - band, thr_out = A.e2 + band;
-        const int cs = A.cell_start[cc], ce = A.cell_start[cc + 1];
-        for (int cb = cs; cb < ce; cb += 32) {
-            const bool bcol_ok = cb + col < ce;
-            const int64_t bcol = bcol_ok ? cb + col : cs;
-            if (PASS == 1 && cb + 31 < row0) continue;  // all columns below every row
+        const int cur_cb = cb;
+        // fetch the next block while this one is computed; one barrier per block
+        advance(ci, cb, cs, ce, false);
+        const bool more = ci < c_end;
+        if (more) fetch(cb, cs, ce);
+        const int cur = buf;
+        buf ^= 1;
+        do {  // compute this block ("break" ends it; the barrier below always runs)
+            if (!wave_on) break;
+            const double R = fmax((double)rmax_row, (double)A.cell_rmax[cc]);
+            const float band = (float)(2.0 * (A.b1 * R * R + A.b2 * R + A.b3));
+            const float thr_in = A.e2 - band, thr_out = A.e2 + band;
+            if (PASS == 0 && cc != last_cc) {  // row thresholds, once per candidate cell
+                last_cc = cc;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    tin_r[r] = 0.5f * (na_r[r] - thr_in);
+                    tout_r[r] = 0.5f * (na_r[r] - thr_out);
+                }
+            }
+            const float nb = sb_na[cur][col];
+            const int32_t aux = sb_aux[cur][col];
+            if (PASS == 1) {
+                // skip the block when every live row and column is in one set
+                int32_t lo_r = INT_MAX, hi_r = -1;
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    if ((rlive >> r) & 1) {
+                        lo_r = min(lo_r, root_r[r]);
+                        hi_r = max(hi_r, root_r[r]);
+                    }
+                if (nb != INF) {
+                    lo_r = min(lo_r, aux);
+                    hi_r = max(hi_r, aux);
+                }
+                for (int o = 32; o >= 1; o >>= 1) {
+                    lo_r = min(lo_r, __shfl_xor(lo_r, o, 64));
+                    hi_r = max(hi_r, __shfl_xor(hi_r, o, 64));
+                }
+                if (lo_r == hi_r || hi_r < 0 || __ballot(nb != INF) == 0) break;
+            }
+            if (PASS == 2 && __ballot(nb != INF) == 0) break;
             f32x16_t acc = {};
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
-                bf16x8_t bhi = hi8[bcol * (KP / 8) + ks * 2 + half];
-                bf16x8_t blo = lo8[bcol * (KP / 8) + ks * 2 + half];
-                if (!bcol_ok) {
-                    bhi = bf16x8_t{};
-                    blo = bf16x8_t{};
-                }
+                const bf16x8_t bhi = sb_hi[cur][col * ROWU + ks * 2 + half];
+                const bf16x8_t blo = sb_lo[cur][col * ROWU + ks * 2 + half];
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[ks], bhi, acc, 0, 0, 0);
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[ks], blo, acc, 0, 0, 0);
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[ks], bhi, acc, 0, 0, 0);
             }
-            const float nb = A.na[bcol];
-            bool col_core = false;
-            int32_t col_label = 0, col_root = 0;
-            if (PASS == 1 || PASS == 2) col_core = bcol_ok && A.core_s[bcol];
-            if (PASS == 2 && col_core) col_label = A.label_s[bcol];
-            if (PASS == 1 && col_core) col_root = mf_find(A.parent, (int32_t)bcol);
+            const int32_t bcol = cur_cb + col;
+            if (PASS == 0) {
+                // est <= thr  <=>  acc >= (|a|^2 - thr)/2 + |b|^2/2; thresholds per row
+                // are set once per candidate cell, so an element costs an add, a
+                // compare and a count; guard-band elements only raise a wave flag
+                const float hnb = 0.5f * nb;
+                uint64_t bandw = 0;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const bool in = acc[r] >= tin_r[r] + hnb;
+                    acc_i[r] += in ? 1 : 0;
+                    bandw |= __ballot(!in && acc[r] >= tout_r[r] + hnb);
+                }
+                if (bandw) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        if (!(acc[r] >= tout_r[r] + hnb) || acc[r] >= tin_r[r] + hnb) continue;
+                        acc_i[r] += exact_pred(A.xs + (int64_t)rp_r[r] * A.d, A.xs + (int64_t)bcol * A.d, A.d,
+                                               A.e2) ? 1 : 0;
+                    }
+                }
+                break;
+            }
+            uint32_t in_m = 0, band_m = 0;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                if (!bcol_ok || !((rvalid >> r) & 1)) continue;
-                const int64_t rp = row0 + 8 * (r >> 2) + 4 * half + (r & 3);
-                if (PASS == 1 && (!col_core || !((rcore >> r) & 1) || bcol <= rp || root_r[r] == col_root))
-                    continue;
-                if (PASS == 2 && (!col_core || ((rcore >> r) & 1) || col_label >= acc_i[r])) continue;
-                const float est = (na_r[r] + nb) - 2.0f * acc[r];
-                bool in = est <= thr_in;
-                if (PASS == 3) {
-                    const bool ex = exact_pred(A.xs + rp * A.d, A.xs + bcol * A.d, A.d, A.e2);
+                const float est = __builtin_fmaf(-2.0f, acc[r], na_r[r] + nb);
+                const bool in = est <= thr_in;
+                in_m |= (uint32_t)in << r;
+                band_m |= (uint32_t)(!in && est <= thr_out) << r;
+            }            if (PASS == 3) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    if (!((rlive >> r) & 1) || nb == INF) continue;
+                    const float est = __builtin_fmaf(-2.0f, acc[r], na_r[r] + nb);
+                    const int64_t rp = rp_r[r];
+                    const bool ex = exact_pred(A.xs + rp * A.d, A.xs + (int64_t)bcol * A.d, A.d, A.e2);
                     double d64 = 0.0;
                     for (int k = 0; k < A.d; ++k) {
-                        const double df = (double)A.xs[rp * A.d + k] - (double)A.xs[bcol * A.d + k];
+                        const double df = (double)A.xs[rp * A.d + k] - (double)A.xs[(int64_t)bcol * A.d + k];
                         d64 += df * df;
                     }
-                    const bool band_pair = !in && est <= thr_out;
-                    const bool filter_wrong = (!band_pair) && (in != ex);
-                    const bool f64_flip = ex != (d64 <= (double)A.e2);
+                    const bool in = (in_m >> r) & 1, band_pair = (band_m >> r) & 1;
                     if (band_pair) atomicAdd(A.audit + 0, 1ull);
-                    if (f64_flip) atomicAdd(A.audit + 1, 1ull);
-                    if (filter_wrong) atomicAdd(A.audit + 2, 1ull);
+                    if (ex != (d64 <= (double)A.e2)) atomicAdd(A.audit + 1, 1ull);
+                    if (!band_pair && in != ex) atomicAdd(A.audit + 2, 1ull);
                     amax = fmaxf(amax, (float)fabs((double)est - d64));
-                    continue;
                 }
-                if (!in && est <= thr_out) in = exact_pred(A.xs + rp * A.d, A.xs + bcol * A.d, A.d, A.e2);
-                if (!in) continue;
-                if (PASS == 0) acc_i[r] += 1;
-                if (PASS == 1) {
-                    mf_unite(A.parent, (int32_t)rp, (int32_t)bcol);
-                    root_r[r] = mf_find(A.parent, (int32_t)rp);
-                    col_root = mf_find(A.parent, (int32_t)bcol);
-                }
-                if (PASS == 2) acc_i[r] = col_label;
+                break;
             }
-        }
+            // guard-band estimates: the reference's own arithmetic decides
+            if (__ballot(band_m != 0) != 0 && band_m) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    if ((band_m >> r) & 1) {
+                        const bool ex = exact_pred(A.xs + (int64_t)rp_r[r] * A.d, A.xs + (int64_t)bcol * A.d,
+                                                   A.d, A.e2);
+                        in_m |= (uint32_t)ex << r;
+                        if (PASS == 0) acc_i[r] += ex ? 1 : 0;
+                    }
+            }
+            if (PASS == 1 && in_m) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    if (!((in_m >> r) & 1) || bcol <= rp_r[r] || root_r[r] == aux) continue;
+                    mf_unite(A.parent, rp_r[r], bcol);
+                    root_r[r] = mf_find(A.parent, rp_r[r]);
+                }
+            }
+            if (PASS == 2 && in_m) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    if (((in_m >> r) & 1) && aux < acc_i[r]) acc_i[r] = aux;
+            }
+        } while (0);
+        if (more) publish(buf);
+        __syncthreads();
     }
+    if (!wave_on) return;
     if (PASS == 3) {
         for (int o = 32; o >= 1; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o, 64));
         if (lane == 0) atomicMax(reinterpret_cast<int *>(A.audit_err), __float_as_int(amax));
@@ -399,13 +580,42 @@ __global__ __launch_bounds__(256) void k_mfma_pass(MfmaArgs A) {
         if (col == 0) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                if (!((rvalid >> r) & 1)) continue;
-                const int64_t rp = row0 + 8 * (r >> 2) + 4 * half + (r & 3);
-                if (PASS == 0) A.nn_s[rp] = acc_i[r];
-                if (PASS == 2 && !((rcore >> r) & 1)) A.best_s[rp] = acc_i[r];
+                if (!((rlive >> r) & 1)) continue;
+                if (PASS == 0) A.nn_s[rp_r[r]] = acc_i[r];
+                if (PASS == 2) A.best_s[rp_r[r]] = acc_i[r];
             }
         }
     }
+}
+
+// Non-core rows for the border pass: compacted list (sorted order, so grouped
+// by cell), per-cell tile counts.
+__global__ __launch_bounds__(256) void k_noncore_flag(const uint8_t *__restrict__ core_s, int64_t n,
+                                                      int32_t *__restrict__ f) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) f[i] = !core_s[i];
+}
+__global__ __launch_bounds__(256) void k_noncore_list(const int32_t *__restrict__ f, const int32_t *__restrict__ pos,
+                                                      int64_t n, int32_t *__restrict__ list) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && f[i]) list[pos[i]] = (int32_t)i;
+}
+// Border row tiles of cell c: list entries [pos[cell_start[c]], pos[cell_start[c+1]]).
+__global__ __launch_bounds__(256) void k_btile_count(const int32_t *__restrict__ pos,
+                                                     const int32_t *__restrict__ cell_start, int64_t nc,
+                                                     int32_t *__restrict__ cnt) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nc) cnt[c] = (pos[cell_start[c + 1]] - pos[cell_start[c]] + 31) / 32;
+}
+__global__ __launch_bounds__(256) void k_btiles(const int32_t *__restrict__ pos,
+                                                const int32_t *__restrict__ cell_start,
+                                                const int32_t *__restrict__ toff, int64_t nc,
+                                                int4 *__restrict__ tiles) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int32_t b = pos[cell_start[c]], e = pos[cell_start[c + 1]];
+    int32_t t = toff[c];
+    for (int32_t q = b; q < e; q += 32) tiles[t++] = make_int4((int32_t)c, q, e - q < 32 ? e - q : 32, 0);
 }
 
 // ------------------------------------------------------------- labels --
@@ -480,9 +690,9 @@ bool mfma_supported(int d) { return d >= 4 && d <= 128; }
 
 template <int PASS>
 static void launch_pass(Ctx &c, const MfmaArgs &a, int kp, const char *name) {
-    int64_t waves = a.n_tiles;
-    if (PASS == 3) waves = ceil_div(a.n_tiles, a.audit_stride);
-    const unsigned g = (unsigned)ceil_div(waves, 4);
+    int64_t blocks = a.n_groups;
+    if (PASS == 3) blocks = ceil_div(a.n_groups, a.audit_stride);
+    const unsigned g = (unsigned)blocks;
     if (g == 0) return;
     Launch L(c.prof, c.stream, name);
     switch (kp) {
@@ -497,6 +707,8 @@ static MfmaArgs mfma_args(Ctx &c) {
     MfmaArgs a{};
     a.tiles = c.ctiles.as<int4>();
     a.n_tiles = c.n_tiles;
+    a.groups = c.tiles.as<int4>();
+    a.n_groups = c.n_label_tiles;
     a.cand_off = c.tile_off.as<int32_t>();
     a.cand = c.ranges.as<int32_t>();
     a.cell_start = c.cell_start.as<int32_t>();
@@ -512,7 +724,7 @@ static MfmaArgs mfma_args(Ctx &c) {
     a.e2 = e * e;  // data_process.h:73
     const int kp = kp_for(c.d);
     const double u = std::ldexp(1.0, -24);
-    a.b1 = 2.0 * (3.1 * std::ldexp(1.0, -16) + 3.0 * kp * u) + (2.0 * kp + 8.0) * u;
+    a.b1 = 2.0 * (3.1 * std::ldexp(1.0, -16) + 3.0 * kp * u) + (2.0 * kp + 16.0) * u;
     a.b2 = std::ldexp(1.0, -21) * std::sqrt((double)a.e2 * 1.01);
     a.b3 = (c.d + 3) * u * (double)a.e2;
     return a;
@@ -633,14 +845,31 @@ void mfma_classify(Ctx &c) {
         Launch L(c.prof, s, "mf_row_tiles");
         k_row_tile_count<<<ncb, 256, 0, s>>>(c.cell_start.as<int32_t>(), nc, c.nranges.as<int32_t>());
     }
-    scan_exclusive_i32(c.nranges.as<int32_t>(), c.ctile_off.as<int32_t>(), nc, d_total, c.scratch, s, c.prof);
-    NDBX_HIP(hipMemcpyAsync(&h_total, d_total, 4, hipMemcpyDeviceToHost, s));
+    // the total lands in ctile_off[nc], so a cell's tiles are [toff[c], toff[c+1])
+    scan_exclusive_i32(c.nranges.as<int32_t>(), c.ctile_off.as<int32_t>(), nc, c.ctile_off.as<int32_t>() + nc,
+                       c.scratch, s, c.prof);
+    NDBX_HIP(hipMemcpyAsync(&h_total, c.ctile_off.as<int32_t>() + nc, 4, hipMemcpyDeviceToHost, s));
     c.sync();
     c.n_tiles = h_total;
     c.ctiles.ensure(c.n_tiles * 16 + 16);
     {
         Launch L(c.prof, s, "mf_row_tiles");
         k_row_tiles<<<ncb, 256, 0, s>>>(c.cell_start.as<int32_t>(), c.ctile_off.as<int32_t>(), nc, c.ctiles.as<int4>());
+    }
+    // work groups of up to 4 row tiles of one cell
+    {
+        Launch L(c.prof, s, "mf_row_tiles");
+        k_group_count<<<ncb, 256, 0, s>>>(c.ctile_off.as<int32_t>(), nc, c.nranges.as<int32_t>());
+    }
+    c.cid.ensure((nc + 1) * 4);
+    scan_exclusive_i32(c.nranges.as<int32_t>(), c.cid.as<int32_t>(), nc, d_total, c.scratch, s, c.prof);
+    NDBX_HIP(hipMemcpyAsync(&h_total, d_total, 4, hipMemcpyDeviceToHost, s));
+    c.sync();
+    c.n_label_tiles = h_total;
+    c.tiles.ensure(c.n_label_tiles * 16 + 16);
+    {
+        Launch L(c.prof, s, "mf_row_tiles");
+        k_groups<<<ncb, 256, 0, s>>>(c.ctile_off.as<int32_t>(), nc, c.cid.as<int32_t>(), c.tiles.as<int4>());
     }
     c.pair_tests = (int64_t)h_pairs;  // candidate cell pairs
     // count
@@ -662,7 +891,7 @@ void mfma_classify(Ctx &c) {
         fill_u64(reinterpret_cast<uint64_t *>(au), 0, 4, s, c.prof);
         a.audit = au;
         a.audit_err = reinterpret_cast<float *>(au + 3);
-        a.audit_stride = (int)std::max<int64_t>(1, c.n_tiles / 2048);
+        a.audit_stride = (int)std::max<int64_t>(1, c.n_label_tiles / 512);
         launch_pass<3>(c, a, kp, "mf_audit");
         unsigned long long h[4];
         NDBX_HIP(hipMemcpyAsync(h, au, 32, hipMemcpyDeviceToHost, s));
@@ -701,8 +930,49 @@ void mfma_label(Ctx &c) {
     }
     c.cell_label.ensure(n * 4);
     fill_i32(c.cell_label.as<int32_t>(), NDBX_NO_CLUSTER, n, s, c.prof);
+    // border rows: the non-core points, compacted (still grouped by cell)
+    const int64_t nc = c.n_cells;
+    const int ncb = (int)ceil_div(nc, 256);
+    c.flag.ensure((n + 1) * 4);
+    c.cid.ensure((n + 1) * 4);
+    c.cell_of.ensure(n * 4);
+    int32_t *bpos = c.cid.as<int32_t>();
+    {
+        Launch L(c.prof, s, "mf_border_rows");
+        k_noncore_flag<<<nb, 256, 0, s>>>(c.core_s.as<uint8_t>(), n, c.flag.as<int32_t>());
+    }
+    scan_exclusive_i32(c.flag.as<int32_t>(), bpos, n, bpos + n, c.scratch, s, c.prof);
+    int32_t *btoff = c.ctile_off.as<int32_t>(), *bgoff = c.nranges.as<int32_t>();
+    int32_t *bcnt = c.gflag.as<int32_t>();
+    {
+        Launch L(c.prof, s, "mf_border_rows");
+        k_noncore_list<<<nb, 256, 0, s>>>(c.flag.as<int32_t>(), bpos, n, c.cell_of.as<int32_t>());
+        k_btile_count<<<ncb, 256, 0, s>>>(bpos, c.cell_start.as<int32_t>(), nc, bcnt);
+    }
+    scan_exclusive_i32(bcnt, btoff, nc, btoff + nc, c.scratch, s, c.prof);
+    {
+        Launch L(c.prof, s, "mf_border_rows");
+        k_group_count<<<ncb, 256, 0, s>>>(btoff, nc, bcnt);
+    }
+    scan_exclusive_i32(bcnt, bgoff, nc, bgoff + nc, c.scratch, s, c.prof);
+    int32_t h_t[2] = {0, 0};
+    NDBX_HIP(hipMemcpyAsync(&h_t[0], btoff + nc, 4, hipMemcpyDeviceToHost, s));
+    NDBX_HIP(hipMemcpyAsync(&h_t[1], bgoff + nc, 4, hipMemcpyDeviceToHost, s));
+    c.sync();
+    c.keys.ensure((size_t)(h_t[0] + h_t[1]) * 16 + 32);
+    int4 *btiles = reinterpret_cast<int4 *>(c.keys.p), *bgroups = btiles + h_t[0];
+    {
+        Launch L(c.prof, s, "mf_border_rows");
+        k_btiles<<<ncb, 256, 0, s>>>(bpos, c.cell_start.as<int32_t>(), btoff, nc, btiles);
+        k_groups<<<ncb, 256, 0, s>>>(btoff, nc, bgoff, bgroups);
+    }
     a.label_s = c.label_s.as<int32_t>();
     a.best_s = c.cell_label.as<int32_t>();
+    a.rows = c.cell_of.as<int32_t>();
+    a.tiles = btiles;
+    a.n_tiles = h_t[0];
+    a.groups = bgroups;
+    a.n_groups = h_t[1];
     launch_pass<2>(c, a, kp, "mf_border");
     Launch L(c.prof, s, "meta");
     k_mf_out<<<nb, 256, 0, s>>>(c.core_s.as<uint8_t>(), c.label_s.as<int32_t>(), c.cell_label.as<int32_t>(),
