@@ -17,6 +17,9 @@
 namespace ndbx {
 
 constexpr int64_t kBoundsChunk = 1 << 16;
+constexpr int kFineBits = 6;               // 2-D: fine grid of 64 x 64 per cell
+constexpr int kFine = 1 << kFineBits;
+constexpr int kDenseCell = 4096;           // 2-D cells with at least this many points get a fine table
 
 // ------------------------------------------------------------ bounds --
 __global__ __launch_bounds__(256) void k_bounds(const float *__restrict__ x, int d, int dg,
@@ -187,7 +190,8 @@ void grid_plan(Ctx &c) {
     c.cell_bits = lin_bits + inst_bits;
     // 2 bits per dimension of sub-cell position below the cell key, so that
     // within a cell the points are in Morton order of a 4^d sub-grid
-    c.sub_bits = c.cell_bits + 2 * dg <= 62 ? 2 * dg : 0;
+    const int want_sub = dg == 2 ? 2 * kFineBits : 2 * dg;
+    c.sub_bits = c.cell_bits + want_sub <= 62 ? want_sub : 0;
     c.key_bits = c.cell_bits + c.sub_bits;
     c.reach_max = 2;
     if (c.cell_bits > 62) c.path = PATH_BRUTE;
@@ -215,20 +219,27 @@ __global__ __launch_bounds__(256) void k_keys(const float *__restrict__ x, int64
     const int inst = n_inst > 1 ? find_inst(off, n_inst, i) : 0;
     const InstGrid &g = ig[inst];
     uint64_t lin = 0;
-    uint32_t morton = 0;
+    uint32_t sub = 0;
 #pragma unroll
     for (int k = D - 1; k >= 0; --k) {
         const double t = ((double)x[i * D + k] - g.origin[k]) * g.inv_s;
         int64_t ck = (int64_t)floor(t);
         ck = ck < 0 ? 0 : (ck >= g.G[k] ? g.G[k] - 1 : ck);
         lin = lin * (uint64_t)g.G[k] + (uint64_t)ck;
-        int sc = (int)floor((t - (double)ck) * 4.0);
-        sc = sc < 0 ? 0 : (sc > 3 ? 3 : sc);
-        morton |= (uint32_t)(sc & 1) << k;          // bit 0 of each dimension
-        morton |= (uint32_t)(sc >> 1) << (D + k);   // bit 1 of each dimension
+        if (D == 2) {
+            // 64 x 64 fine grid, column-major: fine = fx * 64 + fy
+            int f = (int)floor((t - (double)ck) * (double)kFine);
+            f = f < 0 ? 0 : (f >= kFine ? kFine - 1 : f);
+            sub |= (uint32_t)f << (k == 0 ? kFineBits : 0);
+        } else {
+            int sc = (int)floor((t - (double)ck) * 4.0);
+            sc = sc < 0 ? 0 : (sc > 3 ? 3 : sc);
+            sub |= (uint32_t)(sc & 1) << k;          // bit 0 of each dimension
+            sub |= (uint32_t)(sc >> 1) << (D + k);   // bit 1 of each dimension
+        }
     }
     const uint64_t cell = ((uint64_t)inst << lin_bits) | lin;
-    keys[i] = sub_bits ? (cell << sub_bits) | morton : cell;
+    keys[i] = sub_bits ? (cell << sub_bits) | sub : cell;
     vals[i] = (uint32_t)i;
 }
 
@@ -470,6 +481,119 @@ constexpr int kStage = 256;  // candidates staged in LDS per wave (sparse tiles)
 // wave-uniform (scalar loads) and each lane tests its own point with the
 // reference's arithmetic (data_process.cpp:443-447): ascending d, no
 // contraction, '<='.
+// Every lane's query against candidates flattened from every lane's point
+// range [ps, pe): wave scan of the lengths, 256 candidates at a time staged in
+// LDS with independent loads, then a wave-uniform loop over the stage
+// (broadcast LDS reads).  Adds to the lane's count.
+template <int D>
+__device__ __forceinline__ int32_t staged_count(int ps, int pe, const float (&qx)[D], const float *__restrict__ soa,
+                                                int64_t npad, float e2, float *stg, int lane, int64_t &issued) {
+    const int len = pe > ps ? pe - ps : 0;
+    int incl = len;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t2 = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t2;
+    }
+    const int pre = incl - len;
+    const int total = __builtin_amdgcn_readlane(incl, 63);
+    int32_t cnt = 0;
+    for (int cb = 0; cb < total; cb += kStage) {
+        const int n_here = total - cb < kStage ? total - cb : kStage;
+#pragma unroll
+        for (int u = 0; u < kStage / 64; ++u) {
+            const int gi = cb + u * 64 + lane;
+            const int gc = gi < total ? gi : total - 1;
+            int lo_w = 0, hi_w = 63;
+#pragma unroll
+            for (int it = 0; it < 6; ++it) {
+                const int mid = (lo_w + hi_w + 1) >> 1;
+                if (__shfl(pre, mid, 64) <= gc) lo_w = mid;
+                else hi_w = mid - 1;
+            }
+            const int j = __shfl(ps, lo_w, 64) + gc - __shfl(pre, lo_w, 64);
+            if (gi < total) {
+#pragma unroll
+                for (int k = 0; k < D; ++k) stg[k * kStage + u * 64 + lane] = soa[k * npad + j];
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (int i = 0; i < n_here; ++i) {
+            float l2 = 0.0f;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const float df = qx[k] - stg[k * kStage + i];
+                l2 = __fadd_rn(l2, __fmul_rn(df, df));
+            }
+            cnt += (l2 <= e2) ? 1 : 0;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    issued += total;
+    return cnt;
+}
+
+// Dense 2-D cell B against a full tile: lane fc handles fine column fc.  Rows
+// every point of the tile's box reaches within sqrt(e2 (1 - 2^-18)) are
+// counted from the fine offsets without tests (the float32 predicate accepts
+// any pair that close); rows that some point of the box may reach within
+// sqrt(e2 (1 + 2^-18)) but not all are tested; the rest cannot hold a
+// neighbour.  Column and row bounds are widened by 2^-20 of a cell.
+__device__ __forceinline__ int32_t fine_cell_count(const float (&qx)[2], const float (&lo)[2], const float (&hi)[2],
+                                                   const InstGrid &g, int c0, int c1,
+                                                   const int32_t *__restrict__ fo, const float *__restrict__ soa,
+                                                   int64_t npad, float e2, float *stg, int lane, int64_t &issued) {
+    const double inv_s = g.inv_s, s = 1.0 / inv_s, h = s / kFine, slack = s * 9.5367431640625e-07;
+    const double r2o = search_r2(e2), r2i = (double)e2 * (1.0 - 3.814697265625e-06);
+    const double cx0 = g.origin[0] + (double)c0 * s + (double)lane * h - slack, cx1 = cx0 + h + 2.0 * slack;
+    const double Y0 = g.origin[1] + (double)c1 * s;
+    const double dxmin = fmax(0.0, fmax(cx0 - (double)hi[0], (double)lo[0] - cx1));
+    const double dxmax = fmax(cx1 - (double)lo[0], (double)hi[0] - cx0);
+    int b1s = 0, b1e = 0, b2s = 0, b2e = 0, inner = 0;
+    if (dxmin * dxmin <= r2o) {
+        const double dyo = sqrt(r2o - dxmin * dxmin);
+        int rlo = (int)floor(((double)lo[1] - dyo - Y0 - slack) / h);
+        int rhi = (int)floor(((double)hi[1] + dyo - Y0 + slack) / h);
+        rlo = rlo < 0 ? 0 : rlo;
+        rhi = rhi > kFine - 1 ? kFine - 1 : rhi;
+        if (rlo <= rhi) {
+            const int base = lane * kFine;
+            int ilo = 1, ihi = 0;  // no inner rows
+            if (dxmax * dxmax < r2i) {
+                const double dyi = sqrt(r2i - dxmax * dxmax);
+                const double ya = (double)hi[1] - dyi, yb = (double)lo[1] + dyi;
+                if (ya <= yb) {
+                    int a = (int)ceil((ya - Y0 + slack) / h), b = (int)floor((yb - Y0 - slack) / h) - 1;
+                    a = a < rlo ? rlo : a;
+                    b = b > rhi ? rhi : b;
+                    if (a <= b) {
+                        ilo = a;
+                        ihi = b;
+                    }
+                }
+            }
+            const int p_lo = fo[base + rlo], p_hi = fo[base + rhi + 1];
+            if (ilo <= ihi) {
+                const int p_ilo = fo[base + ilo], p_ihi = fo[base + ihi + 1];
+                inner = p_ihi - p_ilo;
+                b1s = p_lo;
+                b1e = p_ilo;
+                b2s = p_ihi;
+                b2e = p_hi;
+            } else {
+                b1s = p_lo;
+                b1e = p_hi;
+            }
+        }
+    }
+    int bulk = inner;
+    for (int o = 32; o >= 1; o >>= 1) bulk += __shfl_xor(bulk, o, 64);
+    int32_t cnt = bulk;
+    cnt += staged_count<2>(b1s, b1e, qx, soa, npad, e2, stg, lane, issued);
+    cnt += staged_count<2>(b2s, b2e, qx, soa, npad, e2, stg, lane, issued);
+    return cnt;
+}
+
 template <int D>
 __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, int64_t n_tiles,
                                                const int32_t *__restrict__ cell_start,
@@ -479,7 +603,10 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
                                                const int32_t *__restrict__ cell_c, int64_t nc,
                                                const float *__restrict__ soa, int64_t npad,
                                                int32_t *__restrict__ nn_s,
-                                               int64_t *__restrict__ tests) {
+                                               int64_t *__restrict__ tests,
+                                               const int32_t *__restrict__ dense_id,
+                                               const int32_t *__restrict__ fine_off) {
+    __shared__ float stage[4][D][kStage];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
     if (t >= n_tiles) return;
@@ -503,6 +630,34 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
     uint64_t rows = __ballot(pe > ps);
     int32_t cnt = 0;
     int64_t issued = 0;
+    if constexpr (D == 2) {
+        if (tile.w > 16 && dense_id) {
+            // as below, but dense cells of a window take the fine path
+            while (rows) {
+                const int slot = __builtin_ctzll(rows);
+                rows &= rows - 1;
+                const int a = __builtin_amdgcn_readlane(w.x, slot), e = __builtin_amdgcn_readlane(w.y, slot);
+                int run = cell_start[a];
+                for (int B = a; B <= e; ++B) {
+                    const int di = B < e ? dense_id[B] : -1;
+                    if (B < e && di < 0) continue;
+                    const int end = cell_start[B];
+                    int j = run;
+                    issued += end - j;
+                    for (; j + 8 <= end; j += 8) cnt += count_block<D, 8>(qx, soa, npad, j, e2);
+                    for (; j < end; ++j) cnt += count_block<D, 1>(qx, soa, npad, j, e2);
+                    if (B == e) break;
+                    cnt += fine_cell_count(qx, lo, hi, g, cell_c[B], cell_c[nc + B],
+                                           fine_off + (int64_t)di * (kFine * kFine + 1), soa, npad, e2, &stage[wave][0][0],
+                                           lane, issued);
+                    run = cell_start[B + 1];
+                }
+            }
+            if (valid) nn_s[q] = cnt;
+            if (tests && lane == 0) tests[t] = issued * tile.w;
+            return;
+        }
+    }
     if (tile.w > 16) {
         // wave-uniform candidates: every lane tests its own query
         while (rows) {
@@ -521,7 +676,6 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
     // Few queries (sparse tiles): the windows are flattened into one candidate
     // sequence (wave scan of their lengths), staged through LDS 256 at a time
     // with independent loads, and split over lanes = (query qi, slice sub).
-    __shared__ float stage[4][D][kStage];
     const int len = pe - ps;
     int incl = len;
 #pragma unroll
@@ -573,6 +727,40 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
     for (int o = 32; o >= (1 << qb); o >>= 1) cnt += __shfl_xor(cnt, o, 64);
     if (sub == 0 && qvalid) nn_s[tile.z + qi] = cnt;
     if (tests && lane == 0) tests[t] = issued * tile.w;
+}
+
+// Dense 2-D cells (>= kDenseCell points): ids and fine offset tables.
+__global__ __launch_bounds__(256) void k_dense_flags(const int32_t *__restrict__ cell_start, int64_t nc,
+                                                     int32_t *__restrict__ f) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nc) f[c] = cell_start[c + 1] - cell_start[c] >= kDenseCell;
+}
+__global__ __launch_bounds__(256) void k_dense_ids(const int32_t *__restrict__ f, const int32_t *__restrict__ excl,
+                                                   int64_t nc, int32_t *__restrict__ dense_id,
+                                                   int32_t *__restrict__ dense_cell) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    dense_id[c] = f[c] ? excl[c] : -1;
+    if (f[c]) dense_cell[excl[c]] = (int32_t)c;
+}
+// fine_off[k][f]: first point of dense cell k whose fine index is >= f
+// (points of a cell are sorted by fine index), f in [0, 64*64].
+__global__ __launch_bounds__(256) void k_fine_off(const uint64_t *__restrict__ keys,
+                                                  const int32_t *__restrict__ cell_start,
+                                                  const int32_t *__restrict__ dense_cell, int64_t nd,
+                                                  int32_t *__restrict__ fine_off) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nd * (kFine * kFine + 1)) return;
+    const int k = (int)(i / (kFine * kFine + 1)), f = (int)(i - (int64_t)k * (kFine * kFine + 1));
+    const int c = dense_cell[k];
+    int lo = cell_start[c], hi = cell_start[c + 1];
+    const uint64_t mask = (uint64_t)(kFine * kFine - 1);
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((int)(keys[mid] & mask) < f) lo = mid + 1;
+        else hi = mid;
+    }
+    fine_off[(int64_t)k * (kFine * kFine + 1) + f] = lo;
 }
 
 // Sum of int64 values (pair-test statistics): one atomic per block.
@@ -723,6 +911,33 @@ static void grid_classify_d(Ctx &c) {
                                               c.cell_of.as<int32_t>(), c.ctile_off.as<int32_t>(), nc,
                                               c.ctiles.as<int4>());
     }
+    // dense 2-D cells: fine offset tables
+    const int32_t *dense_id = nullptr, *fine_off = nullptr;
+    if (D == 2 && c.sub_bits == 2 * kFineBits) {
+        c.b_parent.ensure((nc + 1) * 4);
+        c.b_rootmin.ensure((nc + 1) * 4);
+        {
+            Launch L(c.prof, s, "dense_tables");
+            k_dense_flags<<<ncb, 256, 0, s>>>(c.cell_start.as<int32_t>(), nc, c.b_parent.as<int32_t>());
+        }
+        scan_exclusive_i32(c.b_parent.as<int32_t>(), c.b_rootmin.as<int32_t>(), nc, d_total, c.scratch, s, c.prof);
+        NDBX_HIP(hipMemcpyAsync(&h_total, d_total, 4, hipMemcpyDeviceToHost, s));
+        c.sync();
+        const int64_t nd = h_total;
+        c.n_dense = nd;
+        if (nd > 0) {
+            c.dense_id.ensure(nc * 4);
+            c.fine_off.ensure((size_t)nd * (kFine * kFine + 1) * 4 + (size_t)nd * 4);
+            int32_t *dense_cell = c.fine_off.as<int32_t>() + nd * (kFine * kFine + 1);
+            Launch L(c.prof, s, "dense_tables");
+            k_dense_ids<<<ncb, 256, 0, s>>>(c.b_parent.as<int32_t>(), c.b_rootmin.as<int32_t>(), nc,
+                                            c.dense_id.as<int32_t>(), dense_cell);
+            k_fine_off<<<(unsigned)ceil_div(nd * (kFine * kFine + 1), 256), 256, 0, s>>>(
+                keys, c.cell_start.as<int32_t>(), dense_cell, nd, c.fine_off.as<int32_t>());
+            dense_id = c.dense_id.as<int32_t>();
+            fine_off = c.fine_off.as<int32_t>();
+        }
+    }
     c.nn_s.ensure(n * 4);
     const bool stats = c.prof.on;
     if (stats) {
@@ -734,7 +949,7 @@ static void grid_classify_d(Ctx &c) {
         k_count<D><<<(unsigned)ceil_div(nct, 4), 256, 0, s>>>(
             c.ctiles.as<int4>(), nct, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig, c.lin_bits,
             c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), nc, c.soa.as<float>(), npad, c.nn_s.as<int32_t>(),
-            stats ? c.tile_tests.as<int64_t>() : nullptr);
+            stats ? c.tile_tests.as<int64_t>() : nullptr, dense_id, fine_off);
     }
     if (stats) {
         Launch L(c.prof, s, "stats");
