@@ -12,14 +12,13 @@
 #include <cmath>
 
 #include "ndbx_internal.h"
+#include "ndbx_dense.h"
 #include "ndbx_tile.h"
 
 namespace ndbx {
 
 constexpr int64_t kBoundsChunk = 1 << 16;
-constexpr int kFineBits = 6;               // 2-D: fine grid of 64 x 64 per cell
-constexpr int kFine = 1 << kFineBits;
-constexpr int kDenseCell = 4096;           // 2-D cells with at least this many points get a fine table
+constexpr int kFineBits = 6;  // 2-D: points of a cell in Morton order of a 64 x 64 sub-grid
 
 // ------------------------------------------------------------ bounds --
 __global__ __launch_bounds__(256) void k_bounds(const float *__restrict__ x, int d, int dg,
@@ -227,10 +226,11 @@ __global__ __launch_bounds__(256) void k_keys(const float *__restrict__ x, int64
         ck = ck < 0 ? 0 : (ck >= g.G[k] ? g.G[k] - 1 : ck);
         lin = lin * (uint64_t)g.G[k] + (uint64_t)ck;
         if (D == 2) {
-            // 64 x 64 fine grid, column-major: fine = fx * 64 + fy
-            int f = (int)floor((t - (double)ck) * (double)kFine);
-            f = f < 0 ? 0 : (f >= kFine ? kFine - 1 : f);
-            sub |= (uint32_t)f << (k == 0 ? kFineBits : 0);
+            // 64 x 64 sub-grid, bits interleaved (dimension 0 in the odd bits)
+            int f = (int)floor((t - (double)ck) * (double)(1 << kFineBits));
+            f = f < 0 ? 0 : (f >= (1 << kFineBits) ? (1 << kFineBits) - 1 : f);
+#pragma unroll
+            for (int b = 0; b < kFineBits; ++b) sub |= (uint32_t)((f >> b) & 1) << (2 * b + (k == 0 ? 1 : 0));
         } else {
             int sc = (int)floor((t - (double)ck) * 4.0);
             sc = sc < 0 ? 0 : (sc > 3 ? 3 : sc);
@@ -398,32 +398,74 @@ __global__ __launch_bounds__(256) void k_stencil(const uint64_t *__restrict__ ce
 // group's points are contiguous in sorted order and are cut into tiles of at
 // most 64 points (one wave each).
 
+// A dense cell is a group of its own, cut into tiles strip by strip.
 __global__ __launch_bounds__(256) void k_group_flags(const uint64_t *__restrict__ cell_key,
                                                      const int32_t *__restrict__ cell_c, int64_t nc,
-                                                     int group_cells, int32_t *__restrict__ gflag) {
+                                                     int group_cells, const int32_t *__restrict__ dense_id,
+                                                     int32_t *__restrict__ gflag) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     bool f = c == 0;
     if (!f) {
         const uint64_t rk = cell_key[c] - (uint64_t)cell_c[c], rk_prev = cell_key[c - 1] - (uint64_t)cell_c[c - 1];
         f = rk != rk_prev || cell_c[c] / group_cells != cell_c[c - 1] / group_cells;
+        if (dense_id) f = f || dense_id[c] >= 0 || dense_id[c - 1] >= 0;
     }
     gflag[c] = f;
+}
+
+// Tiles of the strips of dense cell dc.
+__device__ __forceinline__ int32_t dense_strip_tiles(const DenseCell &dc, const int32_t *__restrict__ tbl, int col) {
+    const int32_t *t = tbl + dc.tbl;
+    const int32_t a = t[col << dc.R], b = t[(col + 1) << dc.R];
+    return (b - a + kTile - 1) / kTile;
 }
 
 // Per cell that opens a group: number of tiles of the group.
 __global__ __launch_bounds__(256) void k_group_tiles(const int32_t *__restrict__ gflag,
                                                      const int32_t *__restrict__ cell_start, int64_t nc,
-                                                     int32_t *__restrict__ ntiles) {
+                                                     const DenseView dv, int32_t *__restrict__ ntiles) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     int32_t t = 0;
     if (gflag[c]) {
-        int64_t e = c + 1;
-        while (e < nc && !gflag[e]) ++e;
-        t = (cell_start[e] - cell_start[c] + kTile - 1) / kTile;
+        const int32_t di = dv.id ? dv.id[c] : -1;
+        if (di >= 0) {
+            const DenseCell dc = dv.info[di];
+            for (int col = 0; col < (1 << dc.L); ++col) t += dense_strip_tiles(dc, dv.tbl, col);
+        } else {
+            int64_t e = c + 1;
+            while (e < nc && !gflag[e]) ++e;
+            t = (cell_start[e] - cell_start[c] + kTile - 1) / kTile;
+        }
     }
     ntiles[c] = t;
+}
+
+// Tiles of dense cell blockIdx.x: 256 threads over its strips (at most
+// 2^kDenseYBits, so 4 per thread), tile offsets from a block scan.
+__global__ __launch_bounds__(256) void k_dense_tile_fill(const DenseView dv, const int32_t *__restrict__ toff,
+                                                         int4 *__restrict__ ctiles) {
+    __shared__ int32_t sum[256];
+    const DenseCell dc = dv.info[blockIdx.x];
+    const int ncol = 1 << dc.L, per = (ncol + 255) / 256;
+    const int c0 = threadIdx.x * per;
+    int32_t mine = 0;
+    for (int col = c0; col < c0 + per && col < ncol; ++col) mine += dense_strip_tiles(dc, dv.tbl, col);
+    sum[threadIdx.x] = mine;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const int32_t v = threadIdx.x >= o ? sum[threadIdx.x - o] : 0;
+        __syncthreads();
+        sum[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int32_t t = toff[dc.cell] + sum[threadIdx.x] - mine;
+    const int32_t *tb = dv.tbl + dc.tbl;
+    for (int col = c0; col < c0 + per && col < ncol; ++col) {
+        const int32_t a = tb[col << dc.R], b = tb[(col + 1) << dc.R];
+        for (int32_t p = a; p < b; p += kTile) ctiles[t++] = make_int4(dc.cell, dc.cell, p, b - p < kTile ? b - p : kTile);
+    }
 }
 
 // Count tiles: (first cell, last cell, first point, points).
@@ -431,9 +473,10 @@ __global__ __launch_bounds__(256) void k_group_tile_fill(const int32_t *__restri
                                                          const int32_t *__restrict__ cell_start,
                                                          const int32_t *__restrict__ cell_of,
                                                          const int32_t *__restrict__ toff, int64_t nc,
+                                                         const int32_t *__restrict__ dense_id,
                                                          int4 *__restrict__ ctiles) {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc || !gflag[c]) return;
+    if (c >= nc || !gflag[c] || (dense_id && dense_id[c] >= 0)) return;
     int64_t e = c + 1;
     while (e < nc && !gflag[e]) ++e;
     const int32_t p1 = cell_start[e];
@@ -472,128 +515,14 @@ __device__ __forceinline__ int32_t count_block(const float (&qx)[D], const float
     return cnt;
 }
 
-constexpr int kStage = 256;  // candidates staged in LDS per wave (sparse tiles)
-
 // One wave per count tile (<= 64 points of a run of cells of one row).  The
 // lanes first compute the candidate window of every stencil row in parallel
 // (tile_window: rows and cells out of reach of the tile's bounding box are
 // dropped), then the wave walks the non-empty windows.  The candidate loop is
 // wave-uniform (scalar loads) and each lane tests its own point with the
 // reference's arithmetic (data_process.cpp:443-447): ascending d, no
-// contraction, '<='.
-// Every lane's query against candidates flattened from every lane's point
-// range [ps, pe): wave scan of the lengths, 256 candidates at a time staged in
-// LDS with independent loads, then a wave-uniform loop over the stage
-// (broadcast LDS reads).  Adds to the lane's count.
-template <int D>
-__device__ __forceinline__ int32_t staged_count(int ps, int pe, const float (&qx)[D], const float *__restrict__ soa,
-                                                int64_t npad, float e2, float *stg, int lane, int64_t &issued) {
-    const int len = pe > ps ? pe - ps : 0;
-    int incl = len;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t2 = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += t2;
-    }
-    const int pre = incl - len;
-    const int total = __builtin_amdgcn_readlane(incl, 63);
-    int32_t cnt = 0;
-    for (int cb = 0; cb < total; cb += kStage) {
-        const int n_here = total - cb < kStage ? total - cb : kStage;
-#pragma unroll
-        for (int u = 0; u < kStage / 64; ++u) {
-            const int gi = cb + u * 64 + lane;
-            const int gc = gi < total ? gi : total - 1;
-            int lo_w = 0, hi_w = 63;
-#pragma unroll
-            for (int it = 0; it < 6; ++it) {
-                const int mid = (lo_w + hi_w + 1) >> 1;
-                if (__shfl(pre, mid, 64) <= gc) lo_w = mid;
-                else hi_w = mid - 1;
-            }
-            const int j = __shfl(ps, lo_w, 64) + gc - __shfl(pre, lo_w, 64);
-            if (gi < total) {
-#pragma unroll
-                for (int k = 0; k < D; ++k) stg[k * kStage + u * 64 + lane] = soa[k * npad + j];
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-        for (int i = 0; i < n_here; ++i) {
-            float l2 = 0.0f;
-#pragma unroll
-            for (int k = 0; k < D; ++k) {
-                const float df = qx[k] - stg[k * kStage + i];
-                l2 = __fadd_rn(l2, __fmul_rn(df, df));
-            }
-            cnt += (l2 <= e2) ? 1 : 0;
-        }
-        __builtin_amdgcn_wave_barrier();
-    }
-    issued += total;
-    return cnt;
-}
-
-// Dense 2-D cell B against a full tile: lane fc handles fine column fc.  Rows
-// every point of the tile's box reaches within sqrt(e2 (1 - 2^-18)) are
-// counted from the fine offsets without tests (the float32 predicate accepts
-// any pair that close); rows that some point of the box may reach within
-// sqrt(e2 (1 + 2^-18)) but not all are tested; the rest cannot hold a
-// neighbour.  Column and row bounds are widened by 2^-20 of a cell.
-__device__ __forceinline__ int32_t fine_cell_count(const float (&qx)[2], const float (&lo)[2], const float (&hi)[2],
-                                                   const InstGrid &g, int c0, int c1,
-                                                   const int32_t *__restrict__ fo, const float *__restrict__ soa,
-                                                   int64_t npad, float e2, float *stg, int lane, int64_t &issued) {
-    const double inv_s = g.inv_s, s = 1.0 / inv_s, h = s / kFine, slack = s * 9.5367431640625e-07;
-    const double r2o = search_r2(e2), r2i = (double)e2 * (1.0 - 3.814697265625e-06);
-    const double cx0 = g.origin[0] + (double)c0 * s + (double)lane * h - slack, cx1 = cx0 + h + 2.0 * slack;
-    const double Y0 = g.origin[1] + (double)c1 * s;
-    const double dxmin = fmax(0.0, fmax(cx0 - (double)hi[0], (double)lo[0] - cx1));
-    const double dxmax = fmax(cx1 - (double)lo[0], (double)hi[0] - cx0);
-    int b1s = 0, b1e = 0, b2s = 0, b2e = 0, inner = 0;
-    if (dxmin * dxmin <= r2o) {
-        const double dyo = sqrt(r2o - dxmin * dxmin);
-        int rlo = (int)floor(((double)lo[1] - dyo - Y0 - slack) / h);
-        int rhi = (int)floor(((double)hi[1] + dyo - Y0 + slack) / h);
-        rlo = rlo < 0 ? 0 : rlo;
-        rhi = rhi > kFine - 1 ? kFine - 1 : rhi;
-        if (rlo <= rhi) {
-            const int base = lane * kFine;
-            int ilo = 1, ihi = 0;  // no inner rows
-            if (dxmax * dxmax < r2i) {
-                const double dyi = sqrt(r2i - dxmax * dxmax);
-                const double ya = (double)hi[1] - dyi, yb = (double)lo[1] + dyi;
-                if (ya <= yb) {
-                    int a = (int)ceil((ya - Y0 + slack) / h), b = (int)floor((yb - Y0 - slack) / h) - 1;
-                    a = a < rlo ? rlo : a;
-                    b = b > rhi ? rhi : b;
-                    if (a <= b) {
-                        ilo = a;
-                        ihi = b;
-                    }
-                }
-            }
-            const int p_lo = fo[base + rlo], p_hi = fo[base + rhi + 1];
-            if (ilo <= ihi) {
-                const int p_ilo = fo[base + ilo], p_ihi = fo[base + ihi + 1];
-                inner = p_ihi - p_ilo;
-                b1s = p_lo;
-                b1e = p_ilo;
-                b2s = p_ihi;
-                b2e = p_hi;
-            } else {
-                b1s = p_lo;
-                b1e = p_hi;
-            }
-        }
-    }
-    int bulk = inner;
-    for (int o = 32; o >= 1; o >>= 1) bulk += __shfl_xor(bulk, o, 64);
-    int32_t cnt = bulk;
-    cnt += staged_count<2>(b1s, b1e, qx, soa, npad, e2, stg, lane, issued);
-    cnt += staged_count<2>(b2s, b2e, qx, soa, npad, e2, stg, lane, issued);
-    return cnt;
-}
-
+// contraction, '<='.  2-D tiles whose windows reach a dense cell take that
+// cell through its strip table.
 template <int D>
 __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, int64_t n_tiles,
                                                const int32_t *__restrict__ cell_start,
@@ -604,8 +533,7 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
                                                const float *__restrict__ soa, int64_t npad,
                                                int32_t *__restrict__ nn_s,
                                                int64_t *__restrict__ tests,
-                                               const int32_t *__restrict__ dense_id,
-                                               const int32_t *__restrict__ fine_off) {
+                                               const DenseView dv) {
     __shared__ float stage[4][D][kStage];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
@@ -631,31 +559,34 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
     int32_t cnt = 0;
     int64_t issued = 0;
     if constexpr (D == 2) {
-        if (tile.w > 16 && dense_id) {
-            // as below, but dense cells of a window take the fine path
-            while (rows) {
-                const int slot = __builtin_ctzll(rows);
-                rows &= rows - 1;
-                const int a = __builtin_amdgcn_readlane(w.x, slot), e = __builtin_amdgcn_readlane(w.y, slot);
-                int run = cell_start[a];
-                for (int B = a; B <= e; ++B) {
-                    const int di = B < e ? dense_id[B] : -1;
-                    if (B < e && di < 0) continue;
-                    const int end = cell_start[B];
-                    int j = run;
-                    issued += end - j;
-                    for (; j + 8 <= end; j += 8) cnt += count_block<D, 8>(qx, soa, npad, j, e2);
-                    for (; j < end; ++j) cnt += count_block<D, 1>(qx, soa, npad, j, e2);
-                    if (B == e) break;
-                    cnt += fine_cell_count(qx, lo, hi, g, cell_c[B], cell_c[nc + B],
-                                           fine_off + (int64_t)di * (kFine * kFine + 1), soa, npad, e2, &stage[wave][0][0],
-                                           lane, issued);
-                    run = cell_start[B + 1];
+        if (dv.id) {
+            bool near = false;
+            for (int B = w.x; B < w.y; ++B) near = near || dv.id[B] >= 0;
+            if (tile.w > 16 || __ballot(near) != 0) {
+                // as below, with the dense cells of each window taken by strips
+                while (rows) {
+                    const int slot = __builtin_ctzll(rows);
+                    rows &= rows - 1;
+                    const int a = __builtin_amdgcn_readlane(w.x, slot), e = __builtin_amdgcn_readlane(w.y, slot);
+                    int run = cell_start[a];  // start of the pending run of ordinary cells
+                    for (int B = a; B <= e; ++B) {
+                        const int di = B < e ? dv.id[B] : -1;
+                        if (B < e && di < 0) continue;
+                        const int end = cell_start[B];
+                        int j = run;
+                        issued += end - j;
+                        for (; j + 8 <= end; j += 8) cnt += count_block<D, 8>(qx, soa, npad, j, e2);
+                        for (; j < end; ++j) cnt += count_block<D, 1>(qx, soa, npad, j, e2);
+                        if (B == e) break;
+                        cnt += dense_cell_count(qx, lo, hi, g, cell_c[B], cell_c[nc + B], dv.info[di], dv.tbl, soa,
+                                                npad, e2, &stage[wave][0][0], lane, issued);
+                        run = cell_start[B + 1];
+                    }
                 }
+                if (valid) nn_s[q] = cnt;
+                if (tests && lane == 0) tests[t] = issued * tile.w;
+                return;
             }
-            if (valid) nn_s[q] = cnt;
-            if (tests && lane == 0) tests[t] = issued * tile.w;
-            return;
         }
     }
     if (tile.w > 16) {
@@ -727,40 +658,6 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
     for (int o = 32; o >= (1 << qb); o >>= 1) cnt += __shfl_xor(cnt, o, 64);
     if (sub == 0 && qvalid) nn_s[tile.z + qi] = cnt;
     if (tests && lane == 0) tests[t] = issued * tile.w;
-}
-
-// Dense 2-D cells (>= kDenseCell points): ids and fine offset tables.
-__global__ __launch_bounds__(256) void k_dense_flags(const int32_t *__restrict__ cell_start, int64_t nc,
-                                                     int32_t *__restrict__ f) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < nc) f[c] = cell_start[c + 1] - cell_start[c] >= kDenseCell;
-}
-__global__ __launch_bounds__(256) void k_dense_ids(const int32_t *__restrict__ f, const int32_t *__restrict__ excl,
-                                                   int64_t nc, int32_t *__restrict__ dense_id,
-                                                   int32_t *__restrict__ dense_cell) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    dense_id[c] = f[c] ? excl[c] : -1;
-    if (f[c]) dense_cell[excl[c]] = (int32_t)c;
-}
-// fine_off[k][f]: first point of dense cell k whose fine index is >= f
-// (points of a cell are sorted by fine index), f in [0, 64*64].
-__global__ __launch_bounds__(256) void k_fine_off(const uint64_t *__restrict__ keys,
-                                                  const int32_t *__restrict__ cell_start,
-                                                  const int32_t *__restrict__ dense_cell, int64_t nd,
-                                                  int32_t *__restrict__ fine_off) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nd * (kFine * kFine + 1)) return;
-    const int k = (int)(i / (kFine * kFine + 1)), f = (int)(i - (int64_t)k * (kFine * kFine + 1));
-    const int c = dense_cell[k];
-    int lo = cell_start[c], hi = cell_start[c + 1];
-    const uint64_t mask = (uint64_t)(kFine * kFine - 1);
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if ((int)(keys[mid] & mask) < f) lo = mid + 1;
-        else hi = mid;
-    }
-    fine_off[(int64_t)k * (kFine * kFine + 1) + f] = lo;
 }
 
 // Sum of int64 values (pair-test statistics): one atomic per block.
@@ -845,7 +742,6 @@ static void grid_classify_d(Ctx &c) {
         std::swap(c.vals, c.vals2);
     }
     const uint64_t *keys = c.keys.as<uint64_t>();
-    const uint32_t *perm = c.vals.as<uint32_t>();
 
     c.flag.ensure(n * 4);
     c.cid.ensure(n * 4);
@@ -857,18 +753,23 @@ static void grid_classify_d(Ctx &c) {
     unsigned long long *d_tests = reinterpret_cast<unsigned long long *>(c.misc.as<int32_t>() + 4);
     cell_table(keys, n, c.sub_bits, c.flag.as<int32_t>(), c.cid.as<int32_t>(), c.cell_of.as<int32_t>(),
                c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), d_total, c.scratch, s, c.prof);
-    const int64_t npad = ceil_div(n, 64) * 64;
-    c.soa.ensure((size_t)npad * D * 4);
-    {
-        Launch L(c.prof, s, "gather_soa");
-        k_gather_soa<D><<<nb, 256, 0, s>>>(c.coords, perm, n, npad, c.soa.as<float>());
-    }
     int32_t h_total = 0;
     NDBX_HIP(hipMemcpyAsync(&h_total, d_total, 4, hipMemcpyDeviceToHost, s));
     c.sync();
     const int64_t nc = h_total;
     c.n_cells = nc;
     const int ncb = (int)ceil_div(nc, 256);
+    // dense 2-D cells: strip order and tables (may reorder points inside cells)
+    DenseView dv{nullptr, nullptr, nullptr};
+    c.n_dense = 0;
+    if (D == 2 && c.sub_bits == 2 * kFineBits) dv = dense_prepare(c, ig, n, nc, d_total, s);
+    const uint32_t *perm = c.vals.as<uint32_t>();
+    const int64_t npad = ceil_div(n, 64) * 64;
+    c.soa.ensure((size_t)npad * D * 4);
+    {
+        Launch L(c.prof, s, "gather_soa");
+        k_gather_soa<D><<<nb, 256, 0, s>>>(c.coords, perm, n, npad, c.soa.as<float>());
+    }
 
     // row hash: power of two >= 2 * cells
     uint64_t cap = 1024;
@@ -896,8 +797,9 @@ static void grid_classify_d(Ctx &c) {
     {
         Launch L(c.prof, s, "group_tiles");
         k_group_flags<<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), c.cell_c.as<int32_t>(), nc, c.group_cells,
-                                          c.gflag.as<int32_t>());
-        k_group_tiles<<<ncb, 256, 0, s>>>(c.gflag.as<int32_t>(), c.cell_start.as<int32_t>(), nc, c.flag.as<int32_t>());
+                                          dv.id, c.gflag.as<int32_t>());
+        k_group_tiles<<<ncb, 256, 0, s>>>(c.gflag.as<int32_t>(), c.cell_start.as<int32_t>(), nc, dv,
+                                          c.flag.as<int32_t>());
     }
     scan_exclusive_i32(c.flag.as<int32_t>(), c.ctile_off.as<int32_t>(), nc, d_total, c.scratch, s, c.prof);
     NDBX_HIP(hipMemcpyAsync(&h_total, d_total, 4, hipMemcpyDeviceToHost, s));
@@ -908,35 +810,10 @@ static void grid_classify_d(Ctx &c) {
     {
         Launch L(c.prof, s, "tiles");
         k_group_tile_fill<<<ncb, 256, 0, s>>>(c.gflag.as<int32_t>(), c.cell_start.as<int32_t>(),
-                                              c.cell_of.as<int32_t>(), c.ctile_off.as<int32_t>(), nc,
+                                              c.cell_of.as<int32_t>(), c.ctile_off.as<int32_t>(), nc, dv.id,
                                               c.ctiles.as<int4>());
-    }
-    // dense 2-D cells: fine offset tables
-    const int32_t *dense_id = nullptr, *fine_off = nullptr;
-    if (D == 2 && c.sub_bits == 2 * kFineBits) {
-        c.b_parent.ensure((nc + 1) * 4);
-        c.b_rootmin.ensure((nc + 1) * 4);
-        {
-            Launch L(c.prof, s, "dense_tables");
-            k_dense_flags<<<ncb, 256, 0, s>>>(c.cell_start.as<int32_t>(), nc, c.b_parent.as<int32_t>());
-        }
-        scan_exclusive_i32(c.b_parent.as<int32_t>(), c.b_rootmin.as<int32_t>(), nc, d_total, c.scratch, s, c.prof);
-        NDBX_HIP(hipMemcpyAsync(&h_total, d_total, 4, hipMemcpyDeviceToHost, s));
-        c.sync();
-        const int64_t nd = h_total;
-        c.n_dense = nd;
-        if (nd > 0) {
-            c.dense_id.ensure(nc * 4);
-            c.fine_off.ensure((size_t)nd * (kFine * kFine + 1) * 4 + (size_t)nd * 4);
-            int32_t *dense_cell = c.fine_off.as<int32_t>() + nd * (kFine * kFine + 1);
-            Launch L(c.prof, s, "dense_tables");
-            k_dense_ids<<<ncb, 256, 0, s>>>(c.b_parent.as<int32_t>(), c.b_rootmin.as<int32_t>(), nc,
-                                            c.dense_id.as<int32_t>(), dense_cell);
-            k_fine_off<<<(unsigned)ceil_div(nd * (kFine * kFine + 1), 256), 256, 0, s>>>(
-                keys, c.cell_start.as<int32_t>(), dense_cell, nd, c.fine_off.as<int32_t>());
-            dense_id = c.dense_id.as<int32_t>();
-            fine_off = c.fine_off.as<int32_t>();
-        }
+        if (c.n_dense > 0)
+            k_dense_tile_fill<<<(unsigned)c.n_dense, 256, 0, s>>>(dv, c.ctile_off.as<int32_t>(), c.ctiles.as<int4>());
     }
     c.nn_s.ensure(n * 4);
     const bool stats = c.prof.on;
@@ -949,7 +826,7 @@ static void grid_classify_d(Ctx &c) {
         k_count<D><<<(unsigned)ceil_div(nct, 4), 256, 0, s>>>(
             c.ctiles.as<int4>(), nct, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig, c.lin_bits,
             c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), nc, c.soa.as<float>(), npad, c.nn_s.as<int32_t>(),
-            stats ? c.tile_tests.as<int64_t>() : nullptr, dense_id, fine_off);
+            stats ? c.tile_tests.as<int64_t>() : nullptr, dv);
     }
     if (stats) {
         Launch L(c.prof, s, "stats");

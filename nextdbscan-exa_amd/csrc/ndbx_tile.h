@@ -6,6 +6,8 @@
 
 namespace ndbx {
 
+constexpr int kStage = 256;  // candidates staged in LDS per wave
+
 __device__ __forceinline__ float wave_min(float v) {
     for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));

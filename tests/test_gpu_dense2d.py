@@ -21,7 +21,7 @@ def check(ndbx, x, eps, m):
     nn, lab, meta, path, prof = run(ndbx, x, eps, m)
     nn_o, lab_o, meta_o = O.dbscan(x, eps, m)
     assert path == "grid"
-    assert "dense_tables" in prof, sorted(prof)
+    assert "dense_prepare" in prof, sorted(prof)
     np.testing.assert_array_equal(nn, nn_o)
     np.testing.assert_array_equal(lab, lab_o)
     assert meta == meta_o
