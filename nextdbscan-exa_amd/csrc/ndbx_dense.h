@@ -36,12 +36,70 @@ struct DenseView {
     const int32_t *tbl;
 };
 
+constexpr int kDenseTile = 16;  // points per count tile of a dense cell
+
+// Tests the kStage staged candidates [0, n) (slots past n hold +inf) against
+// the lane's query.  NSUB = 1: every lane owns a query and tests them all
+// (broadcast reads).  NSUB = 4: lane = (query lane & 15, slice lane >> 4) and
+// slice s tests the candidates i with (i / 4) % 4 == s.
+template <int D, int NSUB>
+__device__ __forceinline__ int32_t stage_test(const float (&qx)[D], const float *stg, int n, float e2, int lane) {
+    int32_t cnt = 0;
+    if constexpr (NSUB == 1) {
+        const int n8 = __builtin_amdgcn_readfirstlane((n + 7) & ~7);
+        for (int i = 0; i < n8; i += 8) {
+            float c[D][8];
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const float4 a = *reinterpret_cast<const float4 *>(stg + k * kStage + i);
+                const float4 b = *reinterpret_cast<const float4 *>(stg + k * kStage + i + 4);
+                c[k][0] = a.x, c[k][1] = a.y, c[k][2] = a.z, c[k][3] = a.w;
+                c[k][4] = b.x, c[k][5] = b.y, c[k][6] = b.z, c[k][7] = b.w;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                float l2 = 0.0f;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const float df = qx[k] - c[k][u];
+                    l2 = __fadd_rn(l2, __fmul_rn(df, df));
+                }
+                cnt += (l2 <= e2) ? 1 : 0;
+            }
+        }
+    } else {
+        static_assert(NSUB == 4, "slices");
+        const int sl = 4 * (lane >> 4);
+        const int n32 = __builtin_amdgcn_readfirstlane((n + 31) & ~31);
+        for (int i = 0; i < n32; i += 32) {
+            float c[D][8];
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const float4 a = *reinterpret_cast<const float4 *>(stg + k * kStage + i + sl);
+                const float4 b = *reinterpret_cast<const float4 *>(stg + k * kStage + i + 16 + sl);
+                c[k][0] = a.x, c[k][1] = a.y, c[k][2] = a.z, c[k][3] = a.w;
+                c[k][4] = b.x, c[k][5] = b.y, c[k][6] = b.z, c[k][7] = b.w;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                float l2 = 0.0f;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const float df = qx[k] - c[k][u];
+                    l2 = __fadd_rn(l2, __fmul_rn(df, df));
+                }
+                cnt += (l2 <= e2) ? 1 : 0;
+            }
+        }
+    }
+    return cnt;
+}
+
 // Every lane's query against the candidates of two point ranges per lane,
 // [s1, e1) and [s2, e2_), flattened over the wave (scan of their lengths) and
-// staged through LDS kStage at a time with independent loads; the stage is
-// then read in a wave-uniform loop (broadcast LDS reads).  Returns the lane's
-// neighbour count among them.
-template <int D>
+// staged through LDS kStage at a time with independent loads.  Returns the
+// lane's count (per slice when NSUB = 4).
+template <int D, int NSUB>
 __device__ __forceinline__ int32_t staged_count2(int s1, int e1, int s2, int e2_, const float (&qx)[D],
                                                  const float *__restrict__ soa, int64_t npad, float e2, float *stg,
                                                  int lane, int64_t &issued) {
@@ -75,34 +133,52 @@ __device__ __forceinline__ int32_t staged_count2(int s1, int e1, int s2, int e2_
             const int l1 = __shfl(len1, lo_w, 64);
             const int b1 = __shfl(s1, lo_w, 64), b2 = __shfl(s2, lo_w, 64);
             const int j = off < l1 ? b1 + off : b2 + (off - l1);
-            if (gi < total) {
+            // slots past the end hold +inf, which no query accepts
 #pragma unroll
-                for (int k = 0; k < D; ++k) stg[k * kStage + u * 64 + lane] = soa[k * npad + j];
-            }
+            for (int k = 0; k < D; ++k)
+                stg[k * kStage + u * 64 + lane] = gi < total ? soa[k * npad + j] : __builtin_inff();
         }
         __builtin_amdgcn_wave_barrier();
-        for (int i = 0; i < n_here; ++i) {
-            float l2 = 0.0f;
-#pragma unroll
-            for (int k = 0; k < D; ++k) {
-                const float df = qx[k] - stg[k * kStage + i];
-                l2 = __fadd_rn(l2, __fmul_rn(df, df));
-            }
-            cnt += (l2 <= e2) ? 1 : 0;
-        }
+        cnt += stage_test<D, NSUB>(qx, stg, n_here, e2, lane);
         __builtin_amdgcn_wave_barrier();
     }
     issued += total;
     return cnt;
 }
 
-// Neighbours of every lane's query in dense cell (c0, c1) described by dc,
+// Every lane's query against the contiguous candidates [ps, pe) (wave-uniform),
+// staged kStage at a time with coalesced loads.
+template <int D, int NSUB>
+__device__ __forceinline__ int32_t staged_run(int ps, int pe, const float (&qx)[D], const float *__restrict__ soa,
+                                              int64_t npad, float e2, float *stg, int lane, int64_t &issued) {
+    int32_t cnt = 0;
+    for (int cb = ps; cb < pe; cb += kStage) {
+        const int n_here = pe - cb < kStage ? pe - cb : kStage;
+#pragma unroll
+        for (int u = 0; u < kStage / 64; ++u) {
+            const int j = cb + u * 64 + lane;
+#pragma unroll
+            for (int k = 0; k < D; ++k) stg[k * kStage + u * 64 + lane] = j < pe ? soa[k * npad + j] : __builtin_inff();
+        }
+        __builtin_amdgcn_wave_barrier();
+        cnt += stage_test<D, NSUB>(qx, stg, n_here, e2, lane);
+        __builtin_amdgcn_wave_barrier();
+    }
+    issued += pe > ps ? pe - ps : 0;
+    return cnt;
+}
+
+// Neighbours of the lanes' queries in dense cell (c0, c1) described by dc,
 // for a tile whose box is [lo, hi].  Lanes take the cell's strips 64 at a
-// time; all lanes must call it (wave-uniform arguments).
+// time; all lanes must call it (wave-uniform arguments).  Points counted
+// without tests (the same for every query of the tile) go to `bulk`; the
+// return value is the lane's tested count (per slice when NSUB = 4).
+template <int NSUB>
 __device__ __forceinline__ int32_t dense_cell_count(const float (&qx)[2], const float (&lo)[2], const float (&hi)[2],
                                                     const InstGrid &g, int c0, int c1, const DenseCell dc,
                                                     const int32_t *__restrict__ tbl, const float *__restrict__ soa,
-                                                    int64_t npad, float e2, float *stg, int lane, int64_t &issued) {
+                                                    int64_t npad, float e2, float *stg, int lane, int32_t &bulk,
+                                                    int64_t &issued) {
     const double s = 1.0 / g.inv_s, slack = s * 9.5367431640625e-07;  // 2^-20 cell
     const int ncol = 1 << dc.L, nrow = 1 << dc.R;
     const double hx = s / ncol, hy = s / nrow;
@@ -159,9 +235,9 @@ __device__ __forceinline__ int32_t dense_cell_count(const float (&qx)[2], const 
                 }
             }
         }
-        int bulk = inner;
-        for (int o = 32; o >= 1; o >>= 1) bulk += __shfl_xor(bulk, o, 64);
-        cnt += bulk + staged_count2<2>(b1s, b1e, b2s, b2e, qx, soa, npad, e2, stg, lane, issued);
+        for (int o = 32; o >= 1; o >>= 1) inner += __shfl_xor(inner, o, 64);
+        bulk += inner;
+        cnt += staged_count2<2, NSUB>(b1s, b1e, b2s, b2e, qx, soa, npad, e2, stg, lane, issued);
     }
     return cnt;
 }

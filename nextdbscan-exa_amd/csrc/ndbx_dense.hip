@@ -16,9 +16,11 @@ __global__ __launch_bounds__(256) void k_dense_flags(const int32_t *__restrict__
     if (c < nc) f[c] = cell_start[c + 1] - cell_start[c] >= kDenseCell;
 }
 
-// Strip bits L: 2^L close to sqrt(points) / 8, so a strip of a cell of
-// uniform density holds 64-point runs about as tall as the strip is wide.
-// Table rows: 2^(L+3), at most 2^kDenseYBits.
+// Strip bits L: 2^L close to sqrt(points / 8), so that in a cell of uniform
+// density a tile of kDenseTile = 16 points of a strip is about twice as tall
+// as the strip is wide (the tile's box height and the strip width plus the
+// box width then add about equally to the band of tested rows).  Table rows:
+// 2^(L+2), at most 2^kDenseYBits.
 __global__ __launch_bounds__(256) void k_dense_ids(const int32_t *__restrict__ f, const int32_t *__restrict__ excl,
                                                    const int32_t *__restrict__ cell_start, int64_t nc,
                                                    int32_t *__restrict__ dense_id, DenseCell *__restrict__ info,
@@ -32,9 +34,9 @@ __global__ __launch_bounds__(256) void k_dense_ids(const int32_t *__restrict__ f
     const int k = excl[c];
     dense_id[c] = k;
     const double cnt = (double)(cell_start[c + 1] - cell_start[c]);
-    int L = (int)lrint(log2(sqrt(cnt) / 8.0));
+    int L = (int)lrint(log2(sqrt(cnt / 8.0)));
     L = L < 3 ? 3 : (L > kDenseYBits ? kDenseYBits : L);
-    const int R = L + 3 < kDenseYBits ? L + 3 : kDenseYBits;
+    const int R = L + 2 < kDenseYBits ? L + 2 : kDenseYBits;
     info[k] = DenseCell{(int32_t)c, L, R, 0};
     tbl_size[k] = (1 << (L + R)) + 1;
 }

@@ -418,7 +418,7 @@ __global__ __launch_bounds__(256) void k_group_flags(const uint64_t *__restrict_
 __device__ __forceinline__ int32_t dense_strip_tiles(const DenseCell &dc, const int32_t *__restrict__ tbl, int col) {
     const int32_t *t = tbl + dc.tbl;
     const int32_t a = t[col << dc.R], b = t[(col + 1) << dc.R];
-    return (b - a + kTile - 1) / kTile;
+    return (b - a + kDenseTile - 1) / kDenseTile;
 }
 
 // Per cell that opens a group: number of tiles of the group.
@@ -464,7 +464,8 @@ __global__ __launch_bounds__(256) void k_dense_tile_fill(const DenseView dv, con
     const int32_t *tb = dv.tbl + dc.tbl;
     for (int col = c0; col < c0 + per && col < ncol; ++col) {
         const int32_t a = tb[col << dc.R], b = tb[(col + 1) << dc.R];
-        for (int32_t p = a; p < b; p += kTile) ctiles[t++] = make_int4(dc.cell, dc.cell, p, b - p < kTile ? b - p : kTile);
+        for (int32_t p = a; p < b; p += kDenseTile)
+            ctiles[t++] = make_int4(dc.cell, dc.cell, p, b - p < kDenseTile ? b - p : kDenseTile);
     }
 }
 
@@ -534,7 +535,7 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
                                                int32_t *__restrict__ nn_s,
                                                int64_t *__restrict__ tests,
                                                const DenseView dv) {
-    __shared__ float stage[4][D][kStage];
+    __shared__ __align__(16) float stage[4][D][kStage];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
     if (t >= n_tiles) return;
@@ -542,7 +543,9 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
     const int4 tile = ctiles[t];
     const int cf = tile.x, cl = tile.y;
     const bool valid = lane < tile.w;
-    const int q = tile.z + (valid ? lane : 0);
+    // tiles of dense 2-D cells hold <= 16 points: lane = (query lane & 15, slice lane >> 4)
+    const bool dtile = D == 2 && dv.id && dv.id[cf] >= 0;
+    const int q = tile.z + (dtile ? ((lane & 15) < tile.w ? (lane & 15) : 0) : (valid ? lane : 0));
     float qx[D], lo[D], hi[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -559,11 +562,35 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
     int32_t cnt = 0;
     int64_t issued = 0;
     if constexpr (D == 2) {
+        if (dtile) {
+            int32_t bulk = 0;
+            while (rows) {
+                const int slot = __builtin_ctzll(rows);
+                rows &= rows - 1;
+                const int a = __builtin_amdgcn_readlane(w.x, slot), e = __builtin_amdgcn_readlane(w.y, slot);
+                int run = cell_start[a];  // start of the pending run of ordinary cells
+                for (int B = a; B <= e; ++B) {
+                    const int di = B < e ? dv.id[B] : -1;
+                    if (B < e && di < 0) continue;
+                    cnt += staged_run<D, 4>(run, cell_start[B], qx, soa, npad, e2, &stage[wave][0][0], lane, issued);
+                    if (B == e) break;
+                    cnt += dense_cell_count<4>(qx, lo, hi, g, cell_c[B], cell_c[nc + B], dv.info[di], dv.tbl, soa,
+                                               npad, e2, &stage[wave][0][0], lane, bulk, issued);
+                    run = cell_start[B + 1];
+                }
+            }
+            cnt += __shfl_xor(cnt, 16, 64);
+            cnt += __shfl_xor(cnt, 32, 64);
+            if (valid) nn_s[q] = cnt + bulk;
+            if (tests && lane == 0) tests[t] = issued * tile.w;
+            return;
+        }
         if (dv.id) {
             bool near = false;
             for (int B = w.x; B < w.y; ++B) near = near || dv.id[B] >= 0;
             if (tile.w > 16 || __ballot(near) != 0) {
                 // as below, with the dense cells of each window taken by strips
+                int32_t bulk = 0;
                 while (rows) {
                     const int slot = __builtin_ctzll(rows);
                     rows &= rows - 1;
@@ -578,12 +605,12 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
                         for (; j + 8 <= end; j += 8) cnt += count_block<D, 8>(qx, soa, npad, j, e2);
                         for (; j < end; ++j) cnt += count_block<D, 1>(qx, soa, npad, j, e2);
                         if (B == e) break;
-                        cnt += dense_cell_count(qx, lo, hi, g, cell_c[B], cell_c[nc + B], dv.info[di], dv.tbl, soa,
-                                                npad, e2, &stage[wave][0][0], lane, issued);
+                        cnt += dense_cell_count<1>(qx, lo, hi, g, cell_c[B], cell_c[nc + B], dv.info[di], dv.tbl,
+                                                   soa, npad, e2, &stage[wave][0][0], lane, bulk, issued);
                         run = cell_start[B + 1];
                     }
                 }
-                if (valid) nn_s[q] = cnt;
+                if (valid) nn_s[q] = cnt + bulk;
                 if (tests && lane == 0) tests[t] = issued * tile.w;
                 return;
             }
