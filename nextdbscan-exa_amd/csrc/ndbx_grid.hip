@@ -644,7 +644,7 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
     const int pre = incl - len;
     const int total = __builtin_amdgcn_readlane(incl, 63);
     const int qb = tile.w <= 1 ? 0 : tile.w <= 2 ? 1 : tile.w <= 4 ? 2 : tile.w <= 8 ? 3 : 4;
-    const int qi = lane & ((1 << qb) - 1), sub = lane >> qb, nsub = 64 >> qb;
+    const int qi = lane & ((1 << qb) - 1), sub = lane >> qb;
     const bool qvalid = qi < tile.w;
     float my[D];
 #pragma unroll
@@ -663,21 +663,12 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
                 else hi_w = mid - 1;
             }
             const int j = __shfl(ps, lo_w, 64) + gc - __shfl(pre, lo_w, 64);
-            if (gi < total) {
 #pragma unroll
-                for (int k = 0; k < D; ++k) stage[wave][k][u * 64 + lane] = soa[k * npad + j];
-            }
+            for (int k = 0; k < D; ++k)
+                stage[wave][k][u * 64 + lane] = gi < total ? soa[k * npad + j] : __builtin_inff();
         }
         __builtin_amdgcn_wave_barrier();
-        for (int i = sub; i < n_here; i += nsub) {
-            float l2 = 0.0f;
-#pragma unroll
-            for (int k = 0; k < D; ++k) {
-                const float df = my[k] - stage[wave][k][i];
-                l2 = __fadd_rn(l2, __fmul_rn(df, df));
-            }
-            cnt += (l2 <= e2) ? 1 : 0;
-        }
+        cnt += stage_test_sliced<D>(my, &stage[wave][0][0], n_here, e2, lane, qb);
         __builtin_amdgcn_wave_barrier();
     }
     issued = total;

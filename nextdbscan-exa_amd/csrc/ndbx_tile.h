@@ -18,6 +18,39 @@ __device__ __forceinline__ float wave_max(float v) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
+// Tests the staged candidates [0, n) of a wave's LDS stage (kStage slots per
+// dimension; slots past n hold +inf, which no query accepts) for lanes split
+// as (query lane & (2^qb - 1), slice lane >> qb): slice s takes the groups of
+// four candidates i with (i / 4) % (64 >> qb) == s, one 16-byte read per
+// dimension each.  Returns the lane's count for its slice.
+template <int D>
+__device__ __forceinline__ int32_t stage_test_sliced(const float (&qx)[D], const float *stg, int n, float e2,
+                                                     int lane, int qb) {
+    const int stride = 4 * (64 >> qb);
+    const int npad = __builtin_amdgcn_readfirstlane((n + stride - 1) / stride * stride);
+    const int off = 4 * (lane >> qb);
+    int32_t cnt = 0;
+    for (int i = 0; i < npad; i += stride) {
+        float c[D][4];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const float4 a = *reinterpret_cast<const float4 *>(stg + k * kStage + i + off);
+            c[k][0] = a.x, c[k][1] = a.y, c[k][2] = a.z, c[k][3] = a.w;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            float l2 = 0.0f;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const float df = qx[k] - c[k][u];
+                l2 = __fadd_rn(l2, __fmul_rn(df, df));
+            }
+            cnt += (l2 <= e2) ? 1 : 0;
+        }
+    }
+    return cnt;
+}
+
 // Search radius^2 above anything the float32 predicate can accept: e2 (1 + 2^-18).
 __device__ __forceinline__ double search_r2(float e2) { return (double)e2 * (1.0 + 3.814697265625e-06); }
 
