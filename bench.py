@@ -109,6 +109,10 @@ def main():
     ap.add_argument("--n-inst", type=int, default=4096, help="instances of the batched workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU form comparison")
+    ap.add_argument("--graph", action="store_true",
+                    help="batched workload: also time graph replays (NDBX_GRAPH) and report them as value")
+    ap.add_argument("--autotune", action="store_true",
+                    help="batched workload: tile shape from the autotuner (NDBX_AUTOTUNE)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one device-to-device copy of BYTES (a known-size kernel for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -141,7 +145,8 @@ def main():
         n, d = x.shape
         m = 8
         mins = np.full(n_inst, m, np.int32)
-        ctx = ndbx.Batch(x, off, eps_arr, mins, flags=ndbx.PROFILE)
+        tune = ndbx.AUTOTUNE if args.autotune else 0
+        ctx = ndbx.Batch(x, off, eps_arr, mins, flags=ndbx.PROFILE | tune)
         step = lambda: (ctx.run(), ctx.meta())
         eps = float(eps_arr.mean())
         eps_text = None
@@ -168,6 +173,22 @@ def main():
         extra["plain_gpu"] = {"ms_per_step": plain_ms, "value": n / (plain_ms * 1e-3),
                               "form": "one context per instance, run one after another"}
         extra["instances"] = n_inst
+        if args.graph:
+            # the same batch replayed from a captured graph (first run eager + capture)
+            with ndbx.Batch(x, off, eps_arr, mins, flags=ndbx.GRAPH | tune) as gb:
+                for _ in range(max(1, args.warmup)):
+                    gb.run()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(args.steps):
+                    gb.run()
+                    gmeta = gb.meta()
+                torch.cuda.synchronize()
+                g_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+                gc, replays, reruns = gb.info()
+                assert np.array_equal(gmeta, bmeta), "graph replay differs from the eager batched run"
+            extra["graph"] = {"ms_per_step": g_ms, "value": n / (g_ms * 1e-3), "replays": replays,
+                              "reruns": reruns, "group_cells": gc}
     else:
         x, eps, m = synth.make(args.workload, n=args.n)
         n, d = x.shape
@@ -237,6 +258,12 @@ def main():
                                     "frac": tf / FP32_PEAK_TFLOPS, "flops": flops}
     cpu = None if (args.no_cpu_baseline or eps_text is None) else cpu_baseline(x, eps_text, m, n)
 
+    config_extra = {}
+    if "graph" in extra:
+        # value: graph replays; the profiled eager batched run gives the kernel table
+        config_extra = {"form": "graph replay", "eager_ms_per_step": ms_per_step,
+                        "group_cells": extra["graph"]["group_cells"]}
+        ms_per_step = extra["graph"]["ms_per_step"]
     out = {
         "metric": METRIC,
         "value": n / (ms_per_step * 1e-3),
@@ -251,7 +278,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded numpy generator, nextdbscan-exa_amd/synth.py)",
         "config": {"workload": args.workload, "n": n, "d": d, "eps": float(eps), "min_pts": m, "path": path,
-                   "cells": stats["cells"], "tiles": stats["tiles"], "pair_tests": stats["pair_tests"], "result": list(meta)},
+                   "cells": stats["cells"], "tiles": stats["tiles"], "pair_tests": stats["pair_tests"], "result": list(meta), **config_extra},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,

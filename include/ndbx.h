@@ -48,12 +48,21 @@ enum {
     NDBX_DEVICE_INPUT = 1u << 2,
     /* Record HIP events around every kernel launch; see ndbx_profile_*. */
     NDBX_PROFILE = 1u << 3,
-    /* d >= 16: BF16 MFMA distance filter with an FP32 recheck of guard-band
-     * pairs (decisions identical to the FP32 predicate). */
+    /* Accepted for compatibility; 4 <= d <= 128 always takes the BF16 MFMA
+     * distance filter (FP32 recheck of guard-band pairs, so every decision
+     * is the FP32 predicate's) unless NDBX_FORCE_BRUTE is set. */
     NDBX_MFMA_FILTER = 1u << 4,
-    /* With NDBX_MFMA_FILTER: recheck every filter decision against a float64
-     * distance and record the maximum error and flip count (ndbx_audit). */
-    NDBX_FP64_AUDIT = 1u << 5
+    /* MFMA filter: recheck every filter decision against a float64 distance
+     * and record the maximum error and flip counts (ndbx_audit). */
+    NDBX_FP64_AUDIT = 1u << 5,
+    /* ndbx_batch_run: after the first (eager) run, capture the grid pipeline
+     * into a HIP graph and replay it on later runs.  A replay whose cell or
+     * tile counts exceed the captured capacities, or whose cell geometry
+     * leaves the captured key layout, is rerun eagerly and captured again. */
+    NDBX_GRAPH = 1u << 6,
+    /* ndbx_batch_run: choose the counting tile shape (cells of one row per
+     * tile group) by timing the count kernel on the context's input, once. */
+    NDBX_AUTOTUNE = 1u << 7
 };
 
 /* Cluster index of a point that belongs to no cluster (data_process.h:26). */
@@ -146,6 +155,13 @@ int ndbx_batch_create(ndbx_ctx **ctx, const float *coords, const int64_t *offset
 int ndbx_batch_run(ndbx_ctx *ctx);
 /* Per instance (cores, noise, clusters) triples, 3*n_inst ints. */
 int ndbx_batch_meta(ndbx_ctx *ctx, int32_t *meta3);
+/* New coordinates for the same instances (host memory, n_total*d floats);
+ * not for NDBX_DEVICE_INPUT contexts, whose caller writes its own buffer. */
+int ndbx_batch_set_coords(ndbx_ctx *ctx, const float *coords);
+/* Tile shape in use (cells per counting group) and graph statistics:
+ * replays that completed, and replays rerun eagerly. */
+int ndbx_batch_info(ndbx_ctx *ctx, int *group_cells, int64_t *graph_replays,
+                    int64_t *graph_reruns);
 
 #ifdef __cplusplus
 }

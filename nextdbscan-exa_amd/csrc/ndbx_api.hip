@@ -17,12 +17,14 @@ void Ctx::sync() {
 
 Ctx::~Ctx() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
     prof.destroy();
     DevBuf *bufs[] = {&own_coords, &nn, &labels, &core, &meta, &offsets_d, &inst_d, &bounds_part, &keys, &vals,
                       &keys2, &vals2, &flag, &cid, &cell_of, &cell_start, &cell_key, &soa, &row_keys, &row_vals,
                       &nranges, &ranges, &tile_off, &tiles, &nn_s, &core_s, &cell_core_min, &cell_has_core,
                       &parent, &comp_min, &cell_label, &label_s, &rank, &misc, &scratch, &scratch2, &b_parent,
-                      &b_rootmin, &soa_core, &cell_c, &gflag, &ctile_off, &ctiles, &tile_tests, &dense_id, &dense_info, &dense_tmp, &fine_off};
+                      &b_rootmin, &soa_core, &cell_c, &gflag, &ctile_off, &ctiles, &tile_tests, &dense_id, &dense_info, &dense_tmp, &fine_off, &dcount, &plan_in};
     for (auto *b : bufs) b->release();
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -103,6 +105,123 @@ static void run_label(Ctx &c) {
     c.sync();
     c.meta_valid = true;
     c.labelled = true;
+}
+
+// Tile-shape autotuner: cells per counting group (the run of cells of one
+// row that share tiles) from {1, 2, 4, 8, 16}, chosen by the count kernel's
+// time on the context's own input.  Profiling data recorded so far is kept.
+static void autotune(Ctx &c) {
+    c.sync();
+    Profiler saved_state = c.prof;
+    const bool was_on = c.prof.on;
+    c.prof.pending.clear();
+    c.prof.names.clear();
+    c.prof.ms.clear();
+    c.prof.calls.clear();
+    c.prof.on = true;
+    int best = c.group_cells;
+    double best_ms = 1e300;
+    for (int gc : {1, 2, 4, 8, 16}) {
+        c.group_cells = gc;
+        run_classify(c);
+        c.sync();
+        const int slot = c.prof.slot_of("count");
+        const double ms = c.prof.calls[slot] ? c.prof.ms[slot] : 1e300;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = gc;
+        }
+        c.prof.ms.assign(c.prof.ms.size(), 0.0);
+        c.prof.calls.assign(c.prof.calls.size(), 0);
+    }
+    c.group_cells = best;
+    c.tuned = true;
+    // keep the event pool, restore what the caller had recorded
+    saved_state.pool = c.prof.pool;
+    c.prof = saved_state;
+    c.prof.on = was_on;
+}
+
+// Eager classify + label, then, where the run is capturable (grid path, no
+// dense 2-D cells), a sizing run in capacity mode and the capture of that
+// pipeline into a graph.  Later runs replay the graph; a replay whose counts
+// exceed the capacities or whose geometry leaves the captured key layout is
+// rerun eagerly and captured again.
+static void destroy_graph(Ctx &c) {
+    if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
+    if (c.graph) (void)hipGraphDestroy(c.graph);
+    c.graph_exec = nullptr;
+    c.graph = nullptr;
+}
+
+static void capacity_pipeline(Ctx &c) {
+    fill_i32(c.dcount.as<int32_t>() + 2, 0, 1, c.stream, c.prof);
+    grid_plan(c);
+    grid_classify(c);
+    grid_label(c);
+}
+
+static void read_meta(Ctx &c, int32_t *status) {
+    c.h_meta.assign((size_t)c.n_inst * 3, 0);
+    NDBX_HIP(hipMemcpyAsync(c.h_meta.data(), c.meta.p, c.h_meta.size() * 4, hipMemcpyDeviceToHost, c.stream));
+    if (status) NDBX_HIP(hipMemcpyAsync(status, c.dcount.as<int32_t>() + 2, 4, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+}
+
+static void batch_run_graph(Ctx &c) {
+    if (c.graph_exec) {
+        int32_t status = 0;
+        NDBX_HIP(hipGraphLaunch(c.graph_exec, c.stream));
+        read_meta(c, &status);
+        if (status == 0) {
+            c.classified = c.labelled = c.meta_valid = true;
+            ++c.graph_replays;
+            return;
+        }
+        ++c.graph_reruns;
+        destroy_graph(c);
+    }
+    if ((c.flags & NDBX_AUTOTUNE) && !c.tuned) autotune(c);
+    run_classify(c);
+    run_label(c);
+    if (c.path != PATH_GRID || c.d > 3 || c.n == 0 || c.n_dense > 0) return;
+    c.dcount.ensure(64);
+    c.cap_cells = std::min<int64_t>(c.n, c.n_cells + c.n_cells / 4 + 1024);
+    c.cap_tiles = std::min<int64_t>(c.n + c.cap_cells, c.n_tiles + c.n_tiles / 4 + 1024);
+    const bool prof_on = c.prof.on;
+    try {
+        // sizing run: grows every buffer to its capacity size, same results
+        c.capacity_mode = true;
+        capacity_pipeline(c);
+        int32_t status = 0;
+        read_meta(c, &status);
+        if (status != 0) {
+            c.capacity_mode = false;
+            return;
+        }
+        c.prof.on = false;
+        NDBX_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        g_no_alloc = true;
+        try {
+            capacity_pipeline(c);
+        } catch (...) {
+            g_no_alloc = false;
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(c.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        g_no_alloc = false;
+        NDBX_HIP(hipStreamEndCapture(c.stream, &c.graph));
+        NDBX_HIP(hipGraphInstantiate(&c.graph_exec, c.graph, nullptr, nullptr, 0));
+    } catch (...) {
+        c.capacity_mode = false;
+        c.prof.on = prof_on;
+        destroy_graph(c);
+        throw;
+    }
+    c.capacity_mode = false;
+    c.prof.on = prof_on;
 }
 
 }  // namespace ndbx
@@ -197,8 +316,37 @@ int ndbx_label(ndbx_ctx *h) {
 int ndbx_batch_run(ndbx_ctx *h) {
     return guarded([&] {
         need(h != nullptr, "ctx is null");
-        run_classify(h->c);
-        run_label(h->c);
+        Ctx &c = h->c;
+        if (c.flags & NDBX_GRAPH) {
+            batch_run_graph(c);
+            return;
+        }
+        if ((c.flags & NDBX_AUTOTUNE) && !c.tuned) autotune(c);
+        run_classify(c);
+        run_label(c);
+    });
+}
+
+int ndbx_batch_set_coords(ndbx_ctx *h, const float *coords) {
+    return guarded([&] {
+        need(h != nullptr, "ctx is null");
+        Ctx &c = h->c;
+        need(!(c.flags & NDBX_DEVICE_INPUT), "device input: write the caller's buffer instead");
+        need(c.n == 0 || coords != nullptr, "coords is null");
+        if (c.n == 0) return;
+        NDBX_HIP(hipMemcpyAsync(c.own_coords.p, coords, (size_t)c.n * c.d * sizeof(float), hipMemcpyHostToDevice,
+                                c.stream));
+        c.sync();
+        c.classified = c.labelled = c.meta_valid = false;
+    });
+}
+
+int ndbx_batch_info(ndbx_ctx *h, int *group_cells, int64_t *graph_replays, int64_t *graph_reruns) {
+    return guarded([&] {
+        need(h != nullptr, "ctx is null");
+        if (group_cells) *group_cells = h->c.group_cells;
+        if (graph_replays) *graph_replays = h->c.graph_replays;
+        if (graph_reruns) *graph_reruns = h->c.graph_reruns;
     });
 }
 

@@ -85,8 +85,95 @@ static int bits_for(uint64_t v) {  // bits needed to hold values 0..v-1
     return b;
 }
 
+// Each instance's cell geometry on the device, for captured runs: the same
+// arithmetic as grid_plan below.  An instance the fixed key layout cannot
+// hold (special values, reach > 2, too many cells) sets bit 2 of *status.
+__global__ __launch_bounds__(64) void k_plan(const BoundsPart *__restrict__ parts, const int32_t *__restrict__ first,
+                                             const float *__restrict__ eps, const int32_t *__restrict__ min_pts,
+                                             const int64_t *__restrict__ off, int n_inst, int d, int dg, int lin_bits,
+                                             InstGrid *__restrict__ ig, int32_t *__restrict__ status) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_inst) return;
+    InstGrid g;
+    const float e = eps[i];
+    const float e2 = __fmul_rn(e, e);  // data_process.h:73
+    g.e2 = e2;
+    g.m = min_pts[i];
+    g.point_begin = off[i];
+    g.point_end = off[i + 1];
+    g.core_base = 0;
+    for (int k = 0; k < 3; ++k) {
+        g.origin[k] = 0.0;
+        g.G[k] = 1;
+    }
+    g.inv_s = 1.0;
+    g.reach = 2;
+    bool fail = false;
+    if (g.point_end != g.point_begin) {
+        float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        int64_t special = 0;
+        for (int q = first[i]; q < first[i + 1]; ++q) {
+            special += parts[q].special;
+            for (int k = 0; k < dg; ++k) {
+                lo[k] = fminf(lo[k], parts[q].lo[k]);
+                hi[k] = fmaxf(hi[k], parts[q].hi[k]);
+            }
+        }
+        if (special > 0 || !isfinite(e2) || !(e2 >= 1e-30f)) {
+            fail = true;
+        } else {
+            const double s = sqrt((double)e2 / (double)d) * (1.0 - 3.814697265625e-06);
+            g.inv_s = 1.0 / s;
+            const double rmax = sqrt((double)e2 * (1.0 + 9.5367431640625e-07)) * (1.0 + 9.313225746154785e-10);
+            g.reach = (int32_t)ceil(rmax * g.inv_s);
+            uint64_t total = 1;
+            bool too_big = g.reach > 2;
+            for (int k = 0; k < dg && !too_big; ++k) {
+                g.origin[k] = (double)lo[k];
+                const double span = ((double)hi[k] - g.origin[k]) * g.inv_s;
+                if (!(span < 2097152.0)) {
+                    too_big = true;
+                    break;
+                }
+                g.G[k] = (int64_t)floor(span) + 1;
+                total *= (uint64_t)g.G[k];
+            }
+            fail = too_big || total > (1ull << lin_bits);
+        }
+    }
+    if (fail) {  // one cell for the whole instance: every later index stays in range
+        for (int k = 0; k < 3; ++k) {
+            g.origin[k] = 0.0;
+            g.G[k] = 1;
+        }
+        g.inv_s = 1.0;
+        g.reach = 2;
+    }
+    ig[i] = g;
+    if (fail) atomicOr(status, 2);
+}
+
 // Decides the path and, for the grid, each instance's cell geometry.
 void grid_plan(Ctx &c) {
+    if (c.capacity_mode) {
+        // captured runs keep the path and key layout of the eager run; the
+        // geometry comes from the device
+        const int dg = c.d < 3 ? c.d : 3;
+        const int64_t nch = c.n_chunks;
+        int64_t *d_cb = c.plan_in.as<int64_t>();
+        int32_t *d_first = reinterpret_cast<int32_t *>(d_cb + 2 * nch);
+        c.dcount.ensure(64);
+        {
+            Launch L(c.prof, c.stream, "bounds");
+            k_bounds<<<(unsigned)nch, 256, 0, c.stream>>>(c.coords, c.d, dg, d_cb, d_cb + nch,
+                                                          c.bounds_part.as<BoundsPart>());
+            k_plan<<<(unsigned)ceil_div(c.n_inst, 64), 64, 0, c.stream>>>(
+                c.bounds_part.as<BoundsPart>(), d_first, reinterpret_cast<const float *>(d_first + c.n_inst + 1),
+                d_first + 2 * c.n_inst + 1, c.offsets_d.as<int64_t>(), c.n_inst, c.d, dg, c.lin_bits,
+                c.inst_d.as<InstGrid>(), c.dcount.as<int32_t>() + 2);
+        }
+        return;
+    }
     c.path = PATH_GRID;
     if (c.n == 0) {
         c.h_inst.clear();
@@ -104,11 +191,25 @@ void grid_plan(Ctx &c) {
         }
     }
     const int64_t nch = (int64_t)cb.size();
-    c.misc.ensure((size_t)nch * 2 * sizeof(int64_t));
-    c.bounds_part.ensure((size_t)nch * sizeof(BoundsPart));
-    int64_t *d_cb = c.misc.as<int64_t>();
+    // chunk table, first chunk of each instance, eps and min_pts: kept on the
+    // device for the plan kernel of captured runs
+    std::vector<int32_t> first(c.n_inst + 1, 0);
+    for (int64_t q = nch - 1; q >= 0; --q) first[ci[q]] = (int32_t)q;
+    first[c.n_inst] = (int32_t)nch;
+    for (int i = c.n_inst - 1; i >= 0; --i)
+        if (c.h_offsets[i + 1] == c.h_offsets[i]) first[i] = first[i + 1];
+    const size_t plan_bytes = (size_t)nch * 16 + (size_t)(c.n_inst + 1) * 4 + (size_t)c.n_inst * 8;
+    c.plan_in.ensure(plan_bytes);
+    c.n_chunks = nch;
+    int64_t *d_cb = c.plan_in.as<int64_t>();
+    int32_t *d_first = reinterpret_cast<int32_t *>(d_cb + 2 * nch);
     NDBX_HIP(hipMemcpyAsync(d_cb, cb.data(), nch * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
     NDBX_HIP(hipMemcpyAsync(d_cb + nch, ce.data(), nch * sizeof(int64_t), hipMemcpyHostToDevice, c.stream));
+    NDBX_HIP(hipMemcpyAsync(d_first, first.data(), (c.n_inst + 1) * 4, hipMemcpyHostToDevice, c.stream));
+    NDBX_HIP(hipMemcpyAsync(d_first + c.n_inst + 1, c.h_eps.data(), c.n_inst * 4, hipMemcpyHostToDevice, c.stream));
+    NDBX_HIP(hipMemcpyAsync(d_first + 2 * c.n_inst + 1, c.h_m.data(), c.n_inst * 4, hipMemcpyHostToDevice,
+                            c.stream));
+    c.bounds_part.ensure((size_t)nch * sizeof(BoundsPart));
     {
         Launch L(c.prof, c.stream, "bounds");
         k_bounds<<<(unsigned)nch, 256, 0, c.stream>>>(c.coords, c.d, dg, d_cb, d_cb + nch,
@@ -296,10 +397,11 @@ __device__ __forceinline__ uint64_t row_key_of(uint64_t key, int lin_bits, const
     return key - lin % (uint64_t)ig[inst].G[0];
 }
 
-__global__ __launch_bounds__(256) void k_row_insert(const uint64_t *__restrict__ cell_key, int64_t nc,
+__global__ __launch_bounds__(256) void k_row_insert(const uint64_t *__restrict__ cell_key, const int32_t *__restrict__ ncp,
                                                     const InstGrid *__restrict__ ig, int lin_bits,
                                                     uint64_t *__restrict__ tkeys, int2 *__restrict__ tvals,
                                                     uint64_t mask) {
+    const int64_t nc = *ncp;
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const uint64_t rk = row_key_of(cell_key[c], lin_bits, ig);
@@ -340,11 +442,12 @@ __device__ __forceinline__ bool row_lookup(const uint64_t *__restrict__ tkeys, c
 // of cells [cf, cl] of one row is [slot(cf).a, slot(cl).e).
 template <int D>
 __global__ __launch_bounds__(256) void k_stencil(const uint64_t *__restrict__ cell_key,
-                                                 const int32_t *__restrict__ cell_start, int64_t nc,
+                                                 const int32_t *__restrict__ cell_start, const int32_t *__restrict__ ncp,
                                                  const InstGrid *__restrict__ ig, int lin_bits,
                                                  const uint64_t *__restrict__ tkeys, const int2 *__restrict__ tvals,
                                                  uint64_t mask, int2 *__restrict__ ranges, int rows_per_cell,
                                                  int32_t *__restrict__ cell_c) {
+    const int64_t nc = *ncp;
     // one thread per (cell, stencil row)
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= nc * rows_per_cell) return;
@@ -400,9 +503,10 @@ __global__ __launch_bounds__(256) void k_stencil(const uint64_t *__restrict__ ce
 
 // A dense cell is a group of its own, cut into tiles strip by strip.
 __global__ __launch_bounds__(256) void k_group_flags(const uint64_t *__restrict__ cell_key,
-                                                     const int32_t *__restrict__ cell_c, int64_t nc,
+                                                     const int32_t *__restrict__ cell_c, const int32_t *__restrict__ ncp,
                                                      int group_cells, const int32_t *__restrict__ dense_id,
                                                      int32_t *__restrict__ gflag) {
+    const int64_t nc = *ncp;
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     bool f = c == 0;
@@ -422,11 +526,17 @@ __device__ __forceinline__ int32_t dense_strip_tiles(const DenseCell &dc, const 
 }
 
 // Per cell that opens a group: number of tiles of the group.
+// Entries [nc, ncap) are written as 0, so a scan over ncap entries totals
+// the tiles.
 __global__ __launch_bounds__(256) void k_group_tiles(const int32_t *__restrict__ gflag,
-                                                     const int32_t *__restrict__ cell_start, int64_t nc,
-                                                     const DenseView dv, int32_t *__restrict__ ntiles) {
+                                                     const int32_t *__restrict__ cell_start, const int32_t *__restrict__ ncp,
+                                                     int64_t ncap, const DenseView dv, int32_t *__restrict__ ntiles) {
+    const int64_t nc = *ncp;
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
+    if (c >= nc) {
+        if (c < ncap) ntiles[c] = 0;
+        return;
+    }
     int32_t t = 0;
     if (gflag[c]) {
         const int32_t di = dv.id ? dv.id[c] : -1;
@@ -473,16 +583,17 @@ __global__ __launch_bounds__(256) void k_dense_tile_fill(const DenseView dv, con
 __global__ __launch_bounds__(256) void k_group_tile_fill(const int32_t *__restrict__ gflag,
                                                          const int32_t *__restrict__ cell_start,
                                                          const int32_t *__restrict__ cell_of,
-                                                         const int32_t *__restrict__ toff, int64_t nc,
-                                                         const int32_t *__restrict__ dense_id,
+                                                         const int32_t *__restrict__ toff, const int32_t *__restrict__ ncp,
+                                                         int64_t tcap, const int32_t *__restrict__ dense_id,
                                                          int4 *__restrict__ ctiles) {
+    const int64_t nc = *ncp;
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc || !gflag[c] || (dense_id && dense_id[c] >= 0)) return;
     int64_t e = c + 1;
     while (e < nc && !gflag[e]) ++e;
     const int32_t p1 = cell_start[e];
     int32_t t = toff[c];
-    for (int32_t p = cell_start[c]; p < p1; p += kTile) {
+    for (int32_t p = cell_start[c]; p < p1 && t < tcap; p += kTile) {
         const int32_t len = p1 - p < kTile ? p1 - p : kTile;
         ctiles[t++] = make_int4(cell_of[p], cell_of[p + len - 1], p, len);
     }
@@ -525,16 +636,18 @@ __device__ __forceinline__ int32_t count_block(const float (&qx)[D], const float
 // contraction, '<='.  2-D tiles whose windows reach a dense cell take that
 // cell through its strip table.
 template <int D>
-__global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, int64_t n_tiles,
+__global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, const int32_t *__restrict__ ntp,
                                                const int32_t *__restrict__ cell_start,
                                                const uint64_t *__restrict__ cell_key,
                                                const InstGrid *__restrict__ ig, int lin_bits,
                                                const int2 *__restrict__ ranges, int rows_per_cell,
-                                               const int32_t *__restrict__ cell_c, int64_t nc,
+                                               const int32_t *__restrict__ cell_c, const int32_t *__restrict__ ncp,
                                                const float *__restrict__ soa, int64_t npad,
                                                int32_t *__restrict__ nn_s,
                                                int64_t *__restrict__ tests,
                                                const DenseView dv) {
+    const int64_t nc = *ncp;
+    const int64_t n_tiles = *ntp;
     __shared__ __align__(16) float stage[4][D][kStage];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
@@ -725,6 +838,15 @@ __global__ __launch_bounds__(256) void k_classify_out(const int32_t *__restrict_
     core_s[i] = c;
 }
 
+// A captured run's count above its capacity: clamp it (so every later
+// launch stays in bounds) and flag the run for an eager rerun.
+__global__ void k_clamp_count(int32_t *__restrict__ count, int32_t cap, int32_t *__restrict__ status) {
+    if (threadIdx.x == 0 && *count > cap) {
+        *count = cap;
+        atomicOr(status, 1);
+    }
+}
+
 // ------------------------------------------------------------ pipeline --
 static void upload_inst(Ctx &c) {
     c.inst_d.ensure(sizeof(InstGrid) * c.n_inst);
@@ -740,8 +862,13 @@ static void grid_classify_d(Ctx &c) {
     const int64_t n = c.n;
     const hipStream_t s = c.stream;
     const int nb = (int)ceil_div(n, 256);
-    upload_inst(c);
+    // capture: no host reads; cell and tile counts stay on the device and
+    // every cell / tile launch covers the capacities of the eager run
+    const bool cap_mode = c.capacity_mode;
+    if (!cap_mode) upload_inst(c);
     const InstGrid *ig = c.inst_d.as<InstGrid>();
+    c.dcount.ensure(64);
+    int32_t *ncp = c.dcount.as<int32_t>(), *ntp = ncp + 1, *status = ncp + 2;
 
     c.keys.ensure(n * 8);
     c.vals.ensure(n * 4);
@@ -770,17 +897,26 @@ static void grid_classify_d(Ctx &c) {
     int32_t *d_total = c.misc.as<int32_t>();
     unsigned long long *d_tests = reinterpret_cast<unsigned long long *>(c.misc.as<int32_t>() + 4);
     cell_table(keys, n, c.sub_bits, c.flag.as<int32_t>(), c.cid.as<int32_t>(), c.cell_of.as<int32_t>(),
-               c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), d_total, c.scratch, s, c.prof);
+               c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ncp, c.scratch, s, c.prof);
     int32_t h_total = 0;
-    NDBX_HIP(hipMemcpyAsync(&h_total, d_total, 4, hipMemcpyDeviceToHost, s));
-    c.sync();
-    const int64_t nc = h_total;
-    c.n_cells = nc;
+    int64_t nc;  // cells (eager) or cell capacity (capture)
+    if (cap_mode) {
+        nc = c.cap_cells;
+        k_clamp_count<<<1, 64, 0, s>>>(ncp, (int32_t)nc, status);
+    } else {
+        NDBX_HIP(hipMemcpyAsync(&h_total, ncp, 4, hipMemcpyDeviceToHost, s));
+        c.sync();
+        nc = h_total;
+        c.n_cells = nc;
+    }
     const int ncb = (int)ceil_div(nc, 256);
-    // dense 2-D cells: strip order and tables (may reorder points inside cells)
+    // dense 2-D cells: strip order and tables (may reorder points inside
+    // cells); eager runs only, a captured run counts them as ordinary cells
     DenseView dv{nullptr, nullptr, nullptr};
-    c.n_dense = 0;
-    if (D == 2 && c.sub_bits == 2 * kFineBits) dv = dense_prepare(c, ig, n, nc, d_total, s);
+    if (!cap_mode) {
+        c.n_dense = 0;
+        if (D == 2 && c.sub_bits == 2 * kFineBits) dv = dense_prepare(c, ig, n, nc, d_total, s);
+    }
     const uint32_t *perm = c.vals.as<uint32_t>();
     const int64_t npad = ceil_div(n, 64) * 64;
     c.soa.ensure((size_t)npad * D * 4);
@@ -797,7 +933,7 @@ static void grid_classify_d(Ctx &c) {
     fill_u64(c.row_keys.as<uint64_t>(), kEmpty, (int64_t)cap, s, c.prof);
     {
         Launch L(c.prof, s, "row_insert");
-        k_row_insert<<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), nc, ig, c.lin_bits, c.row_keys.as<uint64_t>(),
+        k_row_insert<<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), ncp, ig, c.lin_bits, c.row_keys.as<uint64_t>(),
                                          c.row_vals.as<int2>(), cap - 1);
     }
     const int rows = c.rows_per_cell = D == 3 ? 25 : D == 2 ? 5 : 1;
@@ -805,7 +941,7 @@ static void grid_classify_d(Ctx &c) {
     c.cell_c.ensure((size_t)nc * 3 * 4);
     {
         Launch L(c.prof, s, "stencil");
-        k_stencil<D><<<(unsigned)ceil_div(nc * rows, 256), 256, 0, s>>>(c.cell_key.as<uint64_t>(), c.cell_start.as<int32_t>(), nc, ig, c.lin_bits,
+        k_stencil<D><<<(unsigned)ceil_div(nc * rows, 256), 256, 0, s>>>(c.cell_key.as<uint64_t>(), c.cell_start.as<int32_t>(), ncp, ig, c.lin_bits,
                                          c.row_keys.as<uint64_t>(), c.row_vals.as<int2>(), cap - 1,
                                          c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>());
     }
@@ -814,27 +950,33 @@ static void grid_classify_d(Ctx &c) {
     c.ctile_off.ensure((nc + 1) * 4);
     {
         Launch L(c.prof, s, "group_tiles");
-        k_group_flags<<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), c.cell_c.as<int32_t>(), nc, c.group_cells,
+        k_group_flags<<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), c.cell_c.as<int32_t>(), ncp, c.group_cells,
                                           dv.id, c.gflag.as<int32_t>());
-        k_group_tiles<<<ncb, 256, 0, s>>>(c.gflag.as<int32_t>(), c.cell_start.as<int32_t>(), nc, dv,
+        k_group_tiles<<<ncb, 256, 0, s>>>(c.gflag.as<int32_t>(), c.cell_start.as<int32_t>(), ncp, nc, dv,
                                           c.flag.as<int32_t>());
     }
-    scan_exclusive_i32(c.flag.as<int32_t>(), c.ctile_off.as<int32_t>(), nc, d_total, c.scratch, s, c.prof);
-    NDBX_HIP(hipMemcpyAsync(&h_total, d_total, 4, hipMemcpyDeviceToHost, s));
-    c.sync();
-    const int64_t nct = h_total;
-    c.n_tiles = nct;
+    scan_exclusive_i32(c.flag.as<int32_t>(), c.ctile_off.as<int32_t>(), nc, ntp, c.scratch, s, c.prof);
+    int64_t nct;  // tiles (eager) or tile capacity (capture)
+    if (cap_mode) {
+        nct = c.cap_tiles;
+        k_clamp_count<<<1, 64, 0, s>>>(ntp, (int32_t)nct, status);
+    } else {
+        NDBX_HIP(hipMemcpyAsync(&h_total, ntp, 4, hipMemcpyDeviceToHost, s));
+        c.sync();
+        nct = h_total;
+        c.n_tiles = nct;
+    }
     c.ctiles.ensure(nct * 16);
     {
         Launch L(c.prof, s, "tiles");
         k_group_tile_fill<<<ncb, 256, 0, s>>>(c.gflag.as<int32_t>(), c.cell_start.as<int32_t>(),
-                                              c.cell_of.as<int32_t>(), c.ctile_off.as<int32_t>(), nc, dv.id,
+                                              c.cell_of.as<int32_t>(), c.ctile_off.as<int32_t>(), ncp, nct, dv.id,
                                               c.ctiles.as<int4>());
-        if (c.n_dense > 0)
+        if (dv.id)
             k_dense_tile_fill<<<(unsigned)c.n_dense, 256, 0, s>>>(dv, c.ctile_off.as<int32_t>(), c.ctiles.as<int4>());
     }
     c.nn_s.ensure(n * 4);
-    const bool stats = c.prof.on;
+    const bool stats = c.prof.on && !cap_mode;
     if (stats) {
         c.tile_tests.ensure(nct * 8);
         fill_u64(reinterpret_cast<uint64_t *>(d_tests), 0, 1, s, c.prof);
@@ -842,8 +984,8 @@ static void grid_classify_d(Ctx &c) {
     {
         Launch L(c.prof, s, "count");
         k_count<D><<<(unsigned)ceil_div(nct, 4), 256, 0, s>>>(
-            c.ctiles.as<int4>(), nct, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig, c.lin_bits,
-            c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), nc, c.soa.as<float>(), npad, c.nn_s.as<int32_t>(),
+            c.ctiles.as<int4>(), ntp, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig, c.lin_bits,
+            c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), ncp, c.soa.as<float>(), npad, c.nn_s.as<int32_t>(),
             stats ? c.tile_tests.as<int64_t>() : nullptr, dv);
     }
     if (stats) {

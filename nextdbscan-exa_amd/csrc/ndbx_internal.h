@@ -33,6 +33,10 @@ struct Error : std::exception {
 
 // ------------------------------------------------------- device buffers --
 // Grow-only device allocation owned by a context.
+// Set while a stream capture is open on this thread: a buffer that would
+// have to grow then raises an error instead of allocating.
+extern thread_local bool g_no_alloc;
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -142,7 +146,19 @@ struct Ctx {
     int64_t n_cells = 0, n_tiles = 0, n_label_tiles = 0, pair_tests = 0, n_dense = 0;
     int lin_bits = 0, cell_bits = 0, sub_bits = 0, key_bits = 0, reach_max = 2, rows_per_cell = 1;
     int group_cells = 8;  // cells of one row per counting group
+    bool tuned = false;   // group_cells chosen by the autotuner
     std::vector<InstGrid> h_inst;
+
+    // captured runs (NDBX_GRAPH): an eager run sizes everything, then the
+    // grid pipeline runs in capacity mode (device-side counts, launches over
+    // the capacities) and is captured into a graph that later runs replay
+    bool capacity_mode = false;
+    int64_t cap_cells = 0, cap_tiles = 0, n_chunks = 0;
+    DevBuf dcount;   // int32: cells, tiles, status (bit 1: a count over capacity, bit 2: plan outside layout)
+    DevBuf plan_in;  // chunk table, first chunk per instance, eps, min_pts
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int64_t graph_replays = 0, graph_reruns = 0;
 
     // brute / mfma path state
     DevBuf b_parent, b_rootmin;

@@ -53,8 +53,12 @@ __device__ void uf_unite(int32_t *parent, int32_t a, int32_t b) {
 __global__ __launch_bounds__(256) void k_core_points(const uint8_t *__restrict__ core_s,
                                                      const uint32_t *__restrict__ perm,
                                                      const int32_t *__restrict__ cell_of, int64_t n,
+                                                     const int32_t *__restrict__ ncp,
                                                      int32_t *__restrict__ core_i32,
                                                      int32_t *__restrict__ cell_core_min) {
+    // cells at or past *ncp exist only in a captured run over its capacity
+    // (flagged for an eager rerun): they are left out
+    const int32_t nc = *ncp;
     const int64_t wave_base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
     if (wave_base >= n) return;  // whole wave past the end
     const int64_t i = wave_base + (threadIdx.x & 63);
@@ -68,14 +72,15 @@ __global__ __launch_bounds__(256) void k_core_points(const uint8_t *__restrict__
     const int32_t c_first = __shfl(c, 0, 64);
     if (__ballot(c != c_first) == 0) {
         for (int o = 32; o >= 1; o >>= 1) val = min(val, __shfl_xor(val, o, 64));
-        if ((threadIdx.x & 63) == 0 && val != INT_MAX) atomicMin(cell_core_min + c, val);
-    } else if (is_core) {
+        if ((threadIdx.x & 63) == 0 && val != INT_MAX && c < nc) atomicMin(cell_core_min + c, val);
+    } else if (is_core && c < nc) {
         atomicMin(cell_core_min + c, val);
     }
 }
 
-__global__ __launch_bounds__(256) void k_has_core(const int32_t *__restrict__ cell_core_min, int64_t nc,
+__global__ __launch_bounds__(256) void k_has_core(const int32_t *__restrict__ cell_core_min, const int32_t *__restrict__ ncp,
                                                   int32_t *__restrict__ cell_has_core) {
+    const int64_t nc = *ncp;
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c < nc) cell_has_core[c] = cell_core_min[c] != INT_MAX;
 }
@@ -146,7 +151,7 @@ constexpr int kMaxStencilCells = 128;  // 25 rows x 5 cells for d <= 3
 
 template <int D>
 __global__ __launch_bounds__(256) void k_union(const int32_t *__restrict__ list, const int32_t *__restrict__ n_list,
-                                               int64_t nc, int phase, const int32_t *__restrict__ cell_start,
+                                               const int32_t *__restrict__ ncp, int phase, const int32_t *__restrict__ cell_start,
                                                const uint64_t *__restrict__ cell_key,
                                                const InstGrid *__restrict__ ig, int lin_bits,
                                                const int2 *__restrict__ ranges, int rows_per_cell,
@@ -154,6 +159,7 @@ __global__ __launch_bounds__(256) void k_union(const int32_t *__restrict__ list,
                                                const float *__restrict__ soa_core, int64_t npad,
                                                const int32_t *__restrict__ cell_has_core,
                                                int32_t *__restrict__ parent) {
+    const int64_t nc = *ncp;
     __shared__ int32_t cand[4][kMaxStencilCells];
     __shared__ int32_t cand_root[4][kMaxStencilCells];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -289,9 +295,10 @@ __global__ __launch_bounds__(256) void k_union(const int32_t *__restrict__ list,
 constexpr int kSmallCell = 8;
 
 __global__ __launch_bounds__(256) void k_cell_lists(const int32_t *__restrict__ cell_has_core,
-                                                    const int32_t *__restrict__ cell_start, int64_t nc,
+                                                    const int32_t *__restrict__ cell_start, const int32_t *__restrict__ ncp,
                                                     int32_t *__restrict__ big, int32_t *__restrict__ small,
                                                     int32_t *__restrict__ counts) {
+    const int64_t nc = *ncp;
     __shared__ int32_t wb[4], ws[4], bb, bs;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool v = c < nc && cell_has_core[c];
@@ -377,8 +384,9 @@ __global__ __launch_bounds__(256) void k_union_small(const int32_t *__restrict__
 
 // Every core cell's root and the lowest core id of each root's set.
 __global__ __launch_bounds__(256) void k_flatten(const int32_t *__restrict__ cell_has_core,
-                                                 const int32_t *__restrict__ cell_core_min, int64_t nc,
+                                                 const int32_t *__restrict__ cell_core_min, const int32_t *__restrict__ ncp,
                                                  int32_t *__restrict__ parent, int32_t *__restrict__ root_min) {
+    const int64_t nc = *ncp;
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc || !cell_has_core[c]) return;
     const int32_t r = uf_find(parent, (int32_t)c);
@@ -397,10 +405,11 @@ __global__ __launch_bounds__(256) void k_core_base(const int32_t *__restrict__ r
 // core of its set.
 __global__ __launch_bounds__(256) void k_cell_label(const int32_t *__restrict__ cell_has_core,
                                                     const uint64_t *__restrict__ cell_key,
-                                                    const InstGrid *__restrict__ ig, int lin_bits, int64_t nc,
+                                                    const InstGrid *__restrict__ ig, int lin_bits, const int32_t *__restrict__ ncp,
                                                     const int32_t *__restrict__ parent,
                                                     const int32_t *__restrict__ root_min,
                                                     const int32_t *__restrict__ rank, int32_t *__restrict__ cell_label) {
+    const int64_t nc = *ncp;
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     if (!cell_has_core[c]) {
@@ -421,17 +430,19 @@ __global__ __launch_bounds__(256) void k_cell_label(const int32_t *__restrict__ 
 // holds a core within eps.  Windows are the count windows of the tile, over
 // the box of its non-core points.
 template <int D>
-__global__ __launch_bounds__(256) void k_border(const int4 *__restrict__ ctiles, int64_t n_tiles,
+__global__ __launch_bounds__(256) void k_border(const int4 *__restrict__ ctiles, const int32_t *__restrict__ ntp,
                                                 const int32_t *__restrict__ cell_start,
                                                 const uint64_t *__restrict__ cell_key,
                                                 const InstGrid *__restrict__ ig, int lin_bits,
                                                 const int2 *__restrict__ ranges, int rows_per_cell,
-                                                const int32_t *__restrict__ cell_c, int64_t nc,
+                                                const int32_t *__restrict__ cell_c, const int32_t *__restrict__ ncp,
                                                 const int32_t *__restrict__ cell_of,
                                                 const float *__restrict__ soa, const float *__restrict__ soa_core,
                                                 int64_t npad, const uint8_t *__restrict__ core_s,
                                                 const int32_t *__restrict__ cell_label,
                                                 const uint32_t *__restrict__ perm, int32_t *__restrict__ labels) {
+    const int64_t nc = *ncp;
+    const int64_t n_tiles = *ntp;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
     if (t >= n_tiles) return;
@@ -552,8 +563,9 @@ __global__ __launch_bounds__(256) void k_meta_points(const uint8_t *__restrict__
 
 __global__ __launch_bounds__(256) void k_meta_cells(const int32_t *__restrict__ cell_has_core,
                                                     const uint64_t *__restrict__ cell_key, int lin_bits,
-                                                    const int32_t *__restrict__ parent, int64_t nc,
+                                                    const int32_t *__restrict__ parent, const int32_t *__restrict__ ncp,
                                                     int32_t *__restrict__ meta) {
+    const int64_t nc = *ncp;
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     if (cell_has_core[c] && parent[c] == (int32_t)c) atomicAdd(meta + 3 * (cell_key[c] >> lin_bits) + 2, 1);
@@ -561,7 +573,10 @@ __global__ __launch_bounds__(256) void k_meta_cells(const int32_t *__restrict__ 
 
 template <int D>
 static void grid_label_d(Ctx &c) {
-    const int64_t n = c.n, nc = c.n_cells;
+    // capacity mode (captured runs): cell and tile counts on the device,
+    // launches over the capacities
+    const int64_t n = c.n, nc = c.capacity_mode ? c.cap_cells : c.n_cells;
+    const int32_t *ncp = c.dcount.as<int32_t>(), *ntp = ncp + 1;
     const hipStream_t s = c.stream;
     const int nb = (int)ceil_div(n, 256), ncb = (int)ceil_div(nc, 256);
     InstGrid *ig = c.inst_d.as<InstGrid>();
@@ -576,8 +591,8 @@ static void grid_label_d(Ctx &c) {
     {
         Launch L(c.prof, s, "core");
         k_core_points<<<nb, 256, 0, s>>>(c.core_s.as<uint8_t>(), c.vals.as<uint32_t>(), c.cell_of.as<int32_t>(), n,
-                                         c.label_s.as<int32_t>(), c.cell_core_min.as<int32_t>());
-        k_has_core<<<ncb, 256, 0, s>>>(c.cell_core_min.as<int32_t>(), nc, c.cell_has_core.as<int32_t>());
+                                         ncp, c.label_s.as<int32_t>(), c.cell_core_min.as<int32_t>());
+        k_has_core<<<ncb, 256, 0, s>>>(c.cell_core_min.as<int32_t>(), ncp, c.cell_has_core.as<int32_t>());
     }
     int32_t *d_total = c.misc.as<int32_t>();
     scan_exclusive_i32(c.label_s.as<int32_t>(), c.rank.as<int32_t>(), n, d_total, c.scratch, s, c.prof);
@@ -606,12 +621,12 @@ static void grid_label_d(Ctx &c) {
     fill_i32(counts, 0, 2, s, c.prof);
     {
         Launch L(c.prof, s, "cell_lists");
-        k_cell_lists<<<ncb, 256, 0, s>>>(c.cell_has_core.as<int32_t>(), c.cell_start.as<int32_t>(), nc, big, small,
+        k_cell_lists<<<ncb, 256, 0, s>>>(c.cell_has_core.as<int32_t>(), c.cell_start.as<int32_t>(), ncp, big, small,
                                          counts);
     }
     for (int phase = 0; phase < 2; ++phase) {
         Launch L(c.prof, s, phase == 0 ? "union_faces" : "union");
-        k_union<D><<<(unsigned)ceil_div(nc, 4), 256, 0, s>>>(big, counts, nc, phase, c.cell_start.as<int32_t>(),
+        k_union<D><<<(unsigned)ceil_div(nc, 4), 256, 0, s>>>(big, counts, ncp, phase, c.cell_start.as<int32_t>(),
                                                              c.cell_key.as<uint64_t>(), ig, c.lin_bits,
                                                              c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(),
                                                              c.soa_core.as<float>(), npad,
@@ -625,21 +640,21 @@ static void grid_label_d(Ctx &c) {
     }
     {
         Launch L(c.prof, s, "flatten");
-        k_flatten<<<ncb, 256, 0, s>>>(c.cell_has_core.as<int32_t>(), c.cell_core_min.as<int32_t>(), nc,
+        k_flatten<<<ncb, 256, 0, s>>>(c.cell_has_core.as<int32_t>(), c.cell_core_min.as<int32_t>(), ncp,
                                       c.parent.as<int32_t>(), c.comp_min.as<int32_t>());
     }
     {
         Launch L(c.prof, s, "cell_label");
-        k_cell_label<<<ncb, 256, 0, s>>>(c.cell_has_core.as<int32_t>(), c.cell_key.as<uint64_t>(), ig, c.lin_bits, nc,
+        k_cell_label<<<ncb, 256, 0, s>>>(c.cell_has_core.as<int32_t>(), c.cell_key.as<uint64_t>(), ig, c.lin_bits, ncp,
                                          c.parent.as<int32_t>(), c.comp_min.as<int32_t>(), c.rank.as<int32_t>(),
                                          c.cell_label.as<int32_t>());
     }
     {
         Launch L(c.prof, s, "border");
-        const int64_t nct = c.n_tiles;
+        const int64_t nct = c.capacity_mode ? c.cap_tiles : c.n_tiles;
         k_border<D><<<(unsigned)ceil_div(nct, 4), 256, 0, s>>>(
-            c.ctiles.as<int4>(), nct, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig, c.lin_bits,
-            c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), nc, c.cell_of.as<int32_t>(), c.soa.as<float>(),
+            c.ctiles.as<int4>(), ntp, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig, c.lin_bits,
+            c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), ncp, c.cell_of.as<int32_t>(), c.soa.as<float>(),
             c.soa_core.as<float>(), npad, c.core_s.as<uint8_t>(), c.cell_label.as<int32_t>(), c.vals.as<uint32_t>(),
             c.labels.as<int32_t>());
     }
@@ -649,7 +664,7 @@ static void grid_label_d(Ctx &c) {
         k_meta_points<<<(unsigned)std::min<int64_t>(ceil_div(n, 256), 2048), 256, 0, s>>>(c.core.as<uint8_t>(), c.labels.as<int32_t>(), c.offsets_d.as<int64_t>(),
                                          c.n_inst, n, c.meta.as<int32_t>());
         k_meta_cells<<<ncb, 256, 0, s>>>(c.cell_has_core.as<int32_t>(), c.cell_key.as<uint64_t>(), c.lin_bits,
-                                         c.parent.as<int32_t>(), nc, c.meta.as<int32_t>());
+                                         c.parent.as<int32_t>(), ncp, c.meta.as<int32_t>());
     }
 }
 

@@ -5,9 +5,12 @@
 namespace ndbx {
 
 // ----------------------------------------------------------- buffers --
+thread_local bool g_no_alloc = false;
+
 void *DevBuf::ensure(size_t want) {
     if (want == 0) want = 256;
     if (want <= bytes) return p;
+    if (g_no_alloc) throw Error(NDBX_EDEVICE, "device buffer would grow inside a graph capture");
     release();
     NDBX_HIP(hipMalloc(&p, want));
     bytes = want;

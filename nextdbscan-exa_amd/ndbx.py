@@ -24,6 +24,8 @@ DEVICE_INPUT = 1 << 2
 PROFILE = 1 << 3
 MFMA_FILTER = 1 << 4
 FP64_AUDIT = 1 << 5
+GRAPH = 1 << 6
+AUTOTUNE = 1 << 7
 PATH_NAMES = {0: "grid", 1: "all-pairs", 2: "mfma-filter"}
 
 # Every entry point the header declares, with its argument types.
@@ -57,6 +59,8 @@ SIGNATURES = {
     "ndbx_batch_create": ([_PP, _P, _P, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_uint32], ctypes.c_int),
     "ndbx_batch_run": ([_P], ctypes.c_int),
     "ndbx_batch_meta": ([_P, _P], ctypes.c_int),
+    "ndbx_batch_set_coords": ([_P, _P], ctypes.c_int),
+    "ndbx_batch_info": ([_P, _IP, _I64P, _I64P], ctypes.c_int),
 }
 
 
@@ -226,6 +230,20 @@ class Batch:
         out = np.empty((self.n_inst, 3), np.int32)
         _check(lib().ndbx_batch_meta(self._h, _ptr(out)))
         return out
+
+    def set_coords(self, coords):
+        """New coordinates for the same instances (same shape)."""
+        x = np.ascontiguousarray(coords, dtype=np.float32).reshape(self.n, self.d)
+        _check(lib().ndbx_batch_set_coords(self._h, _ptr(x) if self.n else None))
+        return self
+
+    def info(self):
+        """(cells per counting group, graph replays, graph replays rerun eagerly)."""
+        gc = ctypes.c_int()
+        rp = ctypes.c_int64()
+        rr = ctypes.c_int64()
+        _check(lib().ndbx_batch_info(self._h, ctypes.byref(gc), ctypes.byref(rp), ctypes.byref(rr)))
+        return gc.value, rp.value, rr.value
 
     def nn(self):
         out = np.empty(self.n, np.int32)
