@@ -31,6 +31,7 @@ sys.path.insert(0, os.path.join(ROOT, "nextdbscan-exa_amd"))
 METRIC = "points/s clustered, plain vs MFMA-filter/batched, exact labels; % HBM/MFMA peak"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+BF16_DENSE_PEAK_TFLOPS = 2500.0
 DEFAULT_WORKLOAD = "blobs3d_10m"
 
 
@@ -151,8 +152,9 @@ def main():
         eps = float(eps_arr.mean())
         eps_text = None
         x_dev = torch.from_numpy(x).to("cuda")
+        n_plain = 0 if args.no_plain else n_inst
         plain = [ndbx.Context(None, eps_arr[i], m, device_ptr=x_dev.data_ptr() + int(off[i]) * 4 * d,
-                              n=int(off[i + 1] - off[i]), d=d) for i in range(n_inst)]
+                              n=int(off[i + 1] - off[i]), d=d) for i in range(n_plain)]
         for c in plain:  # warm: allocate every context's buffers
             c.classify().label()
         torch.cuda.synchronize()
@@ -166,12 +168,13 @@ def main():
         # the plain form must give the batched form's per-instance results
         ctx.run()
         bmeta = ctx.meta()
-        for i in range(0, n_inst, max(1, n_inst // 64)):
+        for i in range(0, n_plain, max(1, n_inst // 64)):
             assert tuple(bmeta[i]) == plain[i].meta(), f"instance {i} differs"
         for c in plain:
             c.close()
-        extra["plain_gpu"] = {"ms_per_step": plain_ms, "value": n / (plain_ms * 1e-3),
-                              "form": "one context per instance, run one after another"}
+        if n_plain:
+            extra["plain_gpu"] = {"ms_per_step": plain_ms, "value": n / (plain_ms * 1e-3),
+                                  "form": "one context per instance, run one after another"}
         extra["instances"] = n_inst
         if args.graph:
             # the same batch replayed from a captured graph (first run eager + capture)
@@ -244,7 +247,20 @@ def main():
         total_ms, launches = prof[dominant]
         avg_s = total_ms / launches * 1e-3
         ab = algorithmic_bytes(dominant, n, d, stats["cells"])
-        if ab is not None:
+        if dominant == "mf_count":
+            # distance filter: F = 2d flops per (ordered) point pair of the
+            # candidate cell pairs; the kernel issues 3 bf16 MFMAs per 16-wide
+            # k step (hi*hi, hi*lo, lo*hi) with k padded to kp
+            kp = 16 if d <= 16 else 32 if d <= 32 else 64 if d <= 64 else 128
+            flops = 2.0 * d * float(stats["pair_tests"])
+            tf = flops / avg_s / 1e12
+            roofline = {"bound": "mfma", "kernel": dominant, "achieved": tf, "peak": BF16_DENSE_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": tf / BF16_DENSE_PEAK_TFLOPS,
+                        "traffic": traffic_from_profiles(dominant, args.workload),
+                        "algorithmic_flops": flops, "avg_ms": avg_s * 1e3,
+                        "issued_mfma_tflops": tf * 3.0 * kp / d,
+                        "issued_frac": tf * 3.0 * kp / d / BF16_DENSE_PEAK_TFLOPS}
+        elif ab is not None:
             achieved = ab / avg_s / 1e9
             roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

@@ -552,30 +552,42 @@ __global__ __launch_bounds__(256) void k_group_tiles(const int32_t *__restrict__
     ntiles[c] = t;
 }
 
-// Tiles of dense cell blockIdx.x: 256 threads over its strips (at most
-// 2^kDenseYBits, so 4 per thread), tile offsets from a block scan.
+// Tiles of dense cell blockIdx.x.  The strips are taken in groups of 32;
+// inside a group the tiles are ordered by (tile index in its strip, strip),
+// so consecutive tiles -- which run at the same time on one XCD -- cover a
+// compact patch (32 strips wide, a few tiles tall) and share most of their
+// candidates in that XCD's L2.
+constexpr int kStripGroup = 32;
+
 __global__ __launch_bounds__(256) void k_dense_tile_fill(const DenseView dv, const int32_t *__restrict__ toff,
                                                          int4 *__restrict__ ctiles) {
-    __shared__ int32_t sum[256];
+    __shared__ int32_t nt[1 << kDenseYBits];
+    __shared__ int32_t gbase[(1 << kDenseYBits) / kStripGroup + 1];
     const DenseCell dc = dv.info[blockIdx.x];
-    const int ncol = 1 << dc.L, per = (ncol + 255) / 256;
-    const int c0 = threadIdx.x * per;
-    int32_t mine = 0;
-    for (int col = c0; col < c0 + per && col < ncol; ++col) mine += dense_strip_tiles(dc, dv.tbl, col);
-    sum[threadIdx.x] = mine;
+    const int ncol = 1 << dc.L;
+    const int G = ncol < kStripGroup ? ncol : kStripGroup, ng = ncol / G;
+    for (int col = threadIdx.x; col < ncol; col += blockDim.x) nt[col] = dense_strip_tiles(dc, dv.tbl, col);
     __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-        const int32_t v = threadIdx.x >= o ? sum[threadIdx.x - o] : 0;
-        __syncthreads();
-        sum[threadIdx.x] += v;
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t run = toff[dc.cell];
+        for (int g = 0; g < ng; ++g) {
+            gbase[g] = run;
+            for (int k = g * G; k < g * G + G; ++k) run += nt[k];
+        }
     }
-    int32_t t = toff[dc.cell] + sum[threadIdx.x] - mine;
+    __syncthreads();
     const int32_t *tb = dv.tbl + dc.tbl;
-    for (int col = c0; col < c0 + per && col < ncol; ++col) {
+    for (int col = threadIdx.x; col < ncol; col += blockDim.x) {
+        const int g0 = col - col % G;
         const int32_t a = tb[col << dc.R], b = tb[(col + 1) << dc.R];
-        for (int32_t p = a; p < b; p += kDenseTile)
-            ctiles[t++] = make_int4(dc.cell, dc.cell, p, b - p < kDenseTile ? b - p : kDenseTile);
+        for (int32_t j = 0; j < nt[col]; ++j) {
+            // tiles before (j, col) in the group: all tiles of index < j, and
+            // tile j of the strips left of col
+            int32_t pos = gbase[g0 / G];
+            for (int k = g0; k < g0 + G; ++k) pos += (nt[k] < j ? nt[k] : j) + (k < col && nt[k] > j ? 1 : 0);
+            const int32_t p = a + j * kDenseTile;
+            ctiles[pos] = make_int4(dc.cell, dc.cell, p, b - p < kDenseTile ? b - p : kDenseTile);
+        }
     }
 }
 

@@ -1,4 +1,4 @@
-// MFMA distance filter for d >= 16 (d <= 128): neighbour counts, union-find
+// MFMA distance filter for 4 <= d <= 128: neighbour counts, union-find
 // and border attachment on 32x32 tiles of bf16 matrix products.
 //
 // 1. Partition: K sampled points are cell centres; every point joins its
@@ -157,16 +157,24 @@ __global__ __launch_bounds__(256) void k_cell_radius(const float *__restrict__ x
 // Candidate cell pairs: |centre_C - centre_C'| <= R_C + R_C' + r (+ slack).
 // Pass 0 counts per cell, pass 1 writes (in no particular order: every use is
 // order-independent).
+// Pass 1 also adds |C| * (points of C's candidate cells) to *pts: the point
+// pairs the filter evaluates (one atomic per cell).
 __global__ __launch_bounds__(256) void k_cell_pairs(const float *__restrict__ cent, const uint64_t *__restrict__ cell_key,
                                                     const double *__restrict__ rad, int64_t nc, int d, double r,
                                                     const int32_t *__restrict__ off, int32_t *__restrict__ cnt_or_list,
-                                                    int pass) {
+                                                    int pass, const int32_t *__restrict__ cell_start,
+                                                    unsigned long long *__restrict__ pts) {
     const int64_t c = (int64_t)blockIdx.x;
     if (c >= nc) return;
     const float *a = cent + (int64_t)cell_key[c] * d;
     __shared__ int32_t cnt;
-    if (threadIdx.x == 0) cnt = 0;
+    __shared__ unsigned long long cand_pts;
+    if (threadIdx.x == 0) {
+        cnt = 0;
+        cand_pts = 0;
+    }
     __syncthreads();
+    unsigned long long mine = 0;
     for (int64_t o = threadIdx.x; o < nc; o += blockDim.x) {
         const float *b = cent + (int64_t)cell_key[o] * d;
         double s = 0.0;
@@ -176,12 +184,19 @@ __global__ __launch_bounds__(256) void k_cell_pairs(const float *__restrict__ ce
         }
         const double lim = rad[c] + rad[o] + r;
         if (s <= lim * lim * (1.0 + 1e-9)) {
-            if (pass == 0) atomicAdd(&cnt, 1);
-            else cnt_or_list[off[c] + atomicAdd(&cnt, 1)] = (int32_t)o;
+            if (pass == 0) {
+                atomicAdd(&cnt, 1);
+            } else {
+                cnt_or_list[off[c] + atomicAdd(&cnt, 1)] = (int32_t)o;
+                mine += (unsigned long long)(cell_start[o + 1] - cell_start[o]);
+            }
         }
     }
+    if (pass == 1 && mine) atomicAdd(&cand_pts, mine);
     __syncthreads();
     if (pass == 0 && threadIdx.x == 0) cnt_or_list[c] = cnt;
+    if (pass == 1 && threadIdx.x == 0 && cand_pts)
+        atomicAdd(pts, cand_pts * (unsigned long long)(cell_start[c + 1] - cell_start[c]));
 }
 
 // Row tiles: (cell, first row, rows) with <= 32 rows of one cell.
@@ -825,7 +840,7 @@ void mfma_classify(Ctx &c) {
         Launch L(c.prof, s, "mf_cell_pairs");
         k_cell_pairs<<<(unsigned)nc, 256, 0, s>>>(cent, c.cell_key.as<uint64_t>(),
                                                   reinterpret_cast<double *>(c.b_rootmin.p), nc, d, r, nullptr,
-                                                  c.nranges.as<int32_t>(), 0);
+                                                  c.nranges.as<int32_t>(), 0, nullptr, nullptr);
     }
     scan_exclusive_i32(c.nranges.as<int32_t>(), c.tile_off.as<int32_t>(), nc, c.tile_off.as<int32_t>() + nc,
                        c.scratch, s, c.prof);
@@ -833,12 +848,17 @@ void mfma_classify(Ctx &c) {
     NDBX_HIP(hipMemcpyAsync(&h_pairs, c.tile_off.as<int32_t>() + nc, 4, hipMemcpyDeviceToHost, s));
     c.sync();
     c.ranges.ensure((size_t)h_pairs * 4 + 4);
+    unsigned long long *d_pts = reinterpret_cast<unsigned long long *>(c.misc.as<int32_t>() + 620);
+    fill_u64(reinterpret_cast<uint64_t *>(d_pts), 0, 1, s, c.prof);
     {
         Launch L(c.prof, s, "mf_cell_pairs");
         k_cell_pairs<<<(unsigned)nc, 256, 0, s>>>(cent, c.cell_key.as<uint64_t>(),
                                                   reinterpret_cast<double *>(c.b_rootmin.p), nc, d, r,
-                                                  c.tile_off.as<int32_t>(), c.ranges.as<int32_t>(), 1);
+                                                  c.tile_off.as<int32_t>(), c.ranges.as<int32_t>(), 1,
+                                                  c.cell_start.as<int32_t>(), d_pts);
     }
+    unsigned long long h_pts = 0;
+    NDBX_HIP(hipMemcpyAsync(&h_pts, d_pts, 8, hipMemcpyDeviceToHost, s));
     // row tiles
     c.ctile_off.ensure((nc + 1) * 4);
     {
@@ -871,7 +891,7 @@ void mfma_classify(Ctx &c) {
         Launch L(c.prof, s, "mf_row_tiles");
         k_groups<<<ncb, 256, 0, s>>>(c.ctile_off.as<int32_t>(), nc, c.cid.as<int32_t>(), c.tiles.as<int4>());
     }
-    c.pair_tests = (int64_t)h_pairs;  // candidate cell pairs
+    c.pair_tests = (int64_t)h_pts;  // point pairs of the candidate cell pairs (read after the syncs above)
     // count
     c.nn_s.ensure(n * 4);
     MfmaArgs a = mfma_args(c);
