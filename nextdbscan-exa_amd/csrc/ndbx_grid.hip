@@ -19,6 +19,7 @@ namespace ndbx {
 
 constexpr int64_t kBoundsChunk = 1 << 16;
 constexpr int kFineBits = 6;  // 2-D: points of a cell in Morton order of a 64 x 64 sub-grid
+constexpr int kPartMin = 64;  // count kernel: cells with fewer points are taken whole, not by parts
 
 // ------------------------------------------------------------ bounds --
 __global__ __launch_bounds__(256) void k_bounds(const float *__restrict__ x, int d, int dg,
@@ -657,7 +658,7 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
                                                const float *__restrict__ soa, int64_t npad,
                                                int32_t *__restrict__ nn_s,
                                                int64_t *__restrict__ tests,
-                                               const DenseView dv) {
+                                               const DenseView dv, const int32_t *__restrict__ part_off) {
     const int64_t nc = *ncp;
     const int64_t n_tiles = *ntp;
     __shared__ __align__(16) float stage[4][D][kStage];
@@ -742,10 +743,77 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
         }
     }
     if (tile.w > 16) {
-        // wave-uniform candidates: every lane tests its own query
+        // wave-uniform candidates: every lane tests its own query.  With a
+        // part table, each cell of a window contributes only the runs of its
+        // halves-per-dimension parts within reach of the tile box.
+        const double r2o = search_r2(e2);
+        // parts pay off when the tile box is within about one cell
+        bool small_box = part_off != nullptr;
+#pragma unroll
+        for (int k = 0; k < D; ++k) small_box = small_box && (double)(hi[k] - lo[k]) * g.inv_s <= 1.0;
         while (rows) {
             const int slot = __builtin_ctzll(rows);
             rows &= rows - 1;
+            if (small_box) {
+                const int a = __builtin_amdgcn_readlane(w.x, slot), e = __builtin_amdgcn_readlane(w.y, slot);
+                // the row's cells share dimensions 1..D-1: their half gaps once
+                const double s_ = 1.0 / g.inv_s, h = 0.5 * s_, slack = s_ * 9.5367431640625e-07;
+                double gy[2] = {0.0, 0.0}, gz[2] = {0.0, 0.0};
+                if (D >= 2) {
+#pragma unroll
+                    for (int k = 1; k < D; ++k) {
+                        const double X = g.origin[k] + (double)cell_c[k * nc + a] * s_;
+#pragma unroll
+                        for (int b = 0; b < 2; ++b) {
+                            const double a0 = X + (double)b * h - slack, a1 = a0 + h + 2.0 * slack;
+                            const double gp = fmax(0.0, fmax(a0 - (double)hi[k], (double)lo[k] - a1));
+                            (k == 1 ? gy : gz)[b] = gp * gp;
+                        }
+                    }
+                }
+                int run = cell_start[a];  // pending run of whole cells
+                for (int B = a; B <= e; ++B) {
+                    const int32_t b0 = cell_start[B];
+                    // cells with few points are taken whole (merged into the run)
+                    if (B < e && cell_start[B + 1] - b0 < kPartMin) continue;
+                    int j = run;
+                    issued += b0 - j;
+                    for (; j + 8 <= b0; j += 8) cnt += count_block<D, 8>(qx, soa, npad, j, e2);
+                    for (; j < b0; ++j) cnt += count_block<D, 1>(qx, soa, npad, j, e2);
+                    if (B == e) break;
+                    const double X = g.origin[0] + (double)cell_c[B] * s_;
+                    double gx[2];
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        const double a0 = X + (double)b * h - slack, a1 = a0 + h + 2.0 * slack;
+                        const double gp = fmax(0.0, fmax(a0 - (double)hi[0], (double)lo[0] - a1));
+                        gx[b] = gp * gp;
+                    }
+                    uint32_t mask = 0;
+#pragma unroll
+                    for (int o = 0; o < (1 << D); ++o) {
+                        double sum = gx[(o >> part_bit<D>(0)) & 1];
+                        if (D >= 2) sum += gy[(o >> part_bit<D>(1)) & 1];
+                        if (D >= 3) sum += gz[(o >> part_bit<D>(2)) & 1];
+                        if (sum <= r2o) mask |= 1u << o;
+                    }
+                    const int32_t *po = part_off + (int64_t)B * 8;
+                    const int32_t b_end = cell_start[B + 1];
+                    while (mask) {
+                        const int o1 = __builtin_ctz(mask);
+                        const uint32_t above = ~((1u << o1) - 1u);
+                        const int o2 = (~mask & above) ? __builtin_ctz(~mask & above) : (1 << D);
+                        mask &= o2 < 32 ? ~((1u << o2) - 1u) : 0u;
+                        int jj = po[o1];
+                        const int end = o2 < (1 << D) ? po[o2] : b_end;
+                        issued += end - jj;
+                        for (; jj + 8 <= end; jj += 8) cnt += count_block<D, 8>(qx, soa, npad, jj, e2);
+                        for (; jj < end; ++jj) cnt += count_block<D, 1>(qx, soa, npad, jj, e2);
+                    }
+                    run = b_end;
+                }
+                continue;
+            }
             const int end = __builtin_amdgcn_readlane(pe, slot);
             int j = __builtin_amdgcn_readlane(ps, slot);
             issued += end - j;
@@ -850,6 +918,28 @@ __global__ __launch_bounds__(256) void k_classify_out(const int32_t *__restrict_
     core_s[i] = c;
 }
 
+// Part table: part_off[8 c + o] = first point of cell c whose part (top D
+// bits of its sub-cell code) is >= o, for o < 2^D.
+template <int D>
+__global__ __launch_bounds__(256) void k_part_off(const uint64_t *__restrict__ keys, int sub_bits,
+                                                  const int32_t *__restrict__ cell_start,
+                                                  const int32_t *__restrict__ ncp, int32_t *__restrict__ part_off) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c = i >> D;
+    const int o = (int)(i & ((1 << D) - 1));
+    if (c >= *ncp) return;
+    int lo = cell_start[c], hi = cell_start[c + 1];
+    if (o > 0) {
+        const uint64_t mask = (1ull << sub_bits) - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if ((int)((keys[mid] & mask) >> (sub_bits - D)) < o) lo = mid + 1;
+            else hi = mid;
+        }
+    }
+    part_off[c * 8 + o] = lo;
+}
+
 // A captured run's count above its capacity: clamp it (so every later
 // launch stays in bounds) and flag the run for an eager rerun.
 __global__ void k_clamp_count(int32_t *__restrict__ count, int32_t cap, int32_t *__restrict__ status) {
@@ -922,6 +1012,15 @@ static void grid_classify_d(Ctx &c) {
         c.n_cells = nc;
     }
     const int ncb = (int)ceil_div(nc, 256);
+    // part table (before the dense re-sort: ordinary cells keep this order)
+    const int32_t *part_off = nullptr;
+    if (c.sub_bits > 0) {
+        c.part_off.ensure((size_t)nc * 8 * 4);
+        Launch L(c.prof, s, "part_off");
+        k_part_off<D><<<(unsigned)ceil_div(nc << D, 256), 256, 0, s>>>(keys, c.sub_bits, c.cell_start.as<int32_t>(),
+                                                                        ncp, c.part_off.as<int32_t>());
+        part_off = c.part_off.as<int32_t>();
+    }
     // dense 2-D cells: strip order and tables (may reorder points inside
     // cells); eager runs only, a captured run counts them as ordinary cells
     DenseView dv{nullptr, nullptr, nullptr};
@@ -998,7 +1097,7 @@ static void grid_classify_d(Ctx &c) {
         k_count<D><<<(unsigned)ceil_div(nct, 4), 256, 0, s>>>(
             c.ctiles.as<int4>(), ntp, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig, c.lin_bits,
             c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), ncp, c.soa.as<float>(), npad, c.nn_s.as<int32_t>(),
-            stats ? c.tile_tests.as<int64_t>() : nullptr, dv);
+            stats ? c.tile_tests.as<int64_t>() : nullptr, dv, part_off);
     }
     if (stats) {
         Launch L(c.prof, s, "stats");

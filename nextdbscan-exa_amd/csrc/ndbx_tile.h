@@ -54,6 +54,41 @@ __device__ __forceinline__ int32_t stage_test_sliced(const float (&qx)[D], const
 // Search radius^2 above anything the float32 predicate can accept: e2 (1 + 2^-18).
 __device__ __forceinline__ double search_r2(float e2) { return (double)e2 * (1.0 + 3.814697265625e-06); }
 
+// Halves of a cell per dimension: the points of a cell are sorted by a
+// Morton code of their sub-cell whose top D bits say, per dimension, whether
+// the point lies in the upper half of the cell, so each of the 2^D parts is a
+// contiguous run.  Part bit of dimension k: k for D = 1, 3; 1 - k for D = 2.
+template <int D>
+__device__ __forceinline__ int part_bit(int k) { return D == 2 ? 1 - k : k; }
+
+// Parts of cell (cc) that can hold a point within the search radius of the
+// tile box [lo, hi] (bounds widened by 2^-20 of a cell), as a 2^D-bit mask.
+template <int D>
+__device__ __forceinline__ uint32_t parts_in_reach(const float (&lo)[D], const float (&hi)[D], const InstGrid &g,
+                                                   const int (&cc)[D], double r2o) {
+    const double s = 1.0 / g.inv_s, h = 0.5 * s, slack = s * 9.5367431640625e-07;
+    double gap2[D][2];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const double X = g.origin[k] + (double)cc[k] * s;
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const double a0 = X + (double)b * h - slack, a1 = a0 + h + 2.0 * slack;
+            const double gp = fmax(0.0, fmax(a0 - (double)hi[k], (double)lo[k] - a1));
+            gap2[k][b] = gp * gp;
+        }
+    }
+    uint32_t mask = 0;
+#pragma unroll
+    for (int o = 0; o < (1 << D); ++o) {
+        double sum = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) sum += gap2[k][(o >> part_bit<D>(k)) & 1];
+        if (sum <= r2o) mask |= 1u << o;
+    }
+    return mask;
+}
+
 // Candidate cells of one stencil row of a tile, computed by lane `slot`
 // (slot < rows_per_cell; other lanes return an empty window).  The tile is a
 // run of cells [cf, cl] of one row; its bounding box is [lo, hi].  Rows
