@@ -443,8 +443,9 @@ __global__ __launch_bounds__(256) void k_mfma_pass(MfmaArgs A) {
     }
     __syncthreads();
     int buf = 0;
-    int last_cc = -1;
-    float tin_r[16], tout_r[16];
+    float mid_r[16];  // PASS 0: (|a|^2 - e2)/2 per row (+inf for rows out of range)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) mid_r[r] = 0.5f * (na_r[r] - A.e2);
     while (ci < c_end) {
         const int cc = A.cand[ci];
         const int cur_cb = cb;
@@ -459,14 +460,7 @@ __global__ __launch_bounds__(256) void k_mfma_pass(MfmaArgs A) {
             const double R = fmax((double)rmax_row, (double)A.cell_rmax[cc]);
             const float band = (float)(2.0 * (A.b1 * R * R + A.b2 * R + A.b3));
             const float thr_in = A.e2 - band, thr_out = A.e2 + band;
-            if (PASS == 0 && cc != last_cc) {  // row thresholds, once per candidate cell
-                last_cc = cc;
-#pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    tin_r[r] = 0.5f * (na_r[r] - thr_in);
-                    tout_r[r] = 0.5f * (na_r[r] - thr_out);
-                }
-            }
+
             const float nb = sb_na[cur][col];
             const int32_t aux = sb_aux[cur][col];
             if (PASS == 1) {
@@ -490,6 +484,10 @@ __global__ __launch_bounds__(256) void k_mfma_pass(MfmaArgs A) {
             }
             if (PASS == 2 && __ballot(nb != INF) == 0) break;
             f32x16_t acc = {};
+            if (PASS == 0) {  // count: the accumulator starts at -|b|^2/2 (this lane's column)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[r] = -0.5f * nb;
+            }
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
                 const bf16x8_t bhi = sb_hi[cur][col * ROWU + ks * 2 + half];
@@ -500,21 +498,24 @@ __global__ __launch_bounds__(256) void k_mfma_pass(MfmaArgs A) {
             }
             const int32_t bcol = cur_cb + col;
             if (PASS == 0) {
-                // est <= thr  <=>  acc >= (|a|^2 - thr)/2 + |b|^2/2; thresholds per row
-                // are set once per candidate cell, so an element costs an add, a
-                // compare and a count; guard-band elements only raise a wave flag
-                const float hnb = 0.5f * nb;
-                uint64_t bandw = 0;
+                // With dv = acc - (|a|^2 - e2)/2 = (e2 - est)/2 (the accumulator
+                // starts at -|b|^2/2) and hb = band/2: dv >= hb is a neighbour,
+                // dv < -hb is not, and |dv| <= hb flags the guard band, which the
+                // exact predicate decides.  An element costs a subtract, two
+                // compares (|.| is free) and a count.
+                const float hb = 0.5f * band;
+                bool inband = false;
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    const bool in = acc[r] >= tin_r[r] + hnb;
-                    acc_i[r] += in ? 1 : 0;
-                    bandw |= __ballot(!in && acc[r] >= tout_r[r] + hnb);
+                    const float dv = acc[r] - mid_r[r];
+                    acc_i[r] += dv >= hb ? 1 : 0;
+                    inband = inband || fabsf(dv) <= hb;
                 }
-                if (bandw) {
+                if (__ballot(inband) != 0) {
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
-                        if (!(acc[r] >= tout_r[r] + hnb) || acc[r] >= tin_r[r] + hnb) continue;
+                        const float dv = acc[r] - mid_r[r];
+                        if (dv >= hb || dv < -hb) continue;
                         acc_i[r] += exact_pred(A.xs + (int64_t)rp_r[r] * A.d, A.xs + (int64_t)bcol * A.d, A.d,
                                                A.e2) ? 1 : 0;
                     }
@@ -739,7 +740,10 @@ static MfmaArgs mfma_args(Ctx &c) {
     a.e2 = e * e;  // data_process.h:73
     const int kp = kp_for(c.d);
     const double u = std::ldexp(1.0, -24);
-    a.b1 = 2.0 * (3.1 * std::ldexp(1.0, -16) + 3.0 * kp * u) + (2.0 * kp + 16.0) * u;
+    // the last terms cover the count pass: its accumulator starts at
+    // -|b|^2/2 (one more rounding of a value <= R^2/2 per MFMA, 3 kp / 16
+    // MFMAs) and it compares acc - (|a|^2 - e2)/2 (two more roundings)
+    a.b1 = 2.0 * (3.1 * std::ldexp(1.0, -16) + 3.0 * kp * u) + (2.0 * kp + 16.0) * u + (3.0 * kp / 16.0 + 4.0) * u;
     a.b2 = std::ldexp(1.0, -21) * std::sqrt((double)a.e2 * 1.01);
     a.b3 = (c.d + 3) * u * (double)a.e2;
     return a;
