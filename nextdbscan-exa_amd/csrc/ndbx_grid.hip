@@ -5,9 +5,9 @@
 // (data_process.cpp:433-454).  It is a filter only: every candidate pair
 // still goes through the reference's float32 predicate, so nn is the same
 // integer.  Cell side s = sqrt(e2/d)*(1-2^-18): two points of one cell are
-// always within eps under that predicate (used by the label pass, never by
-// counting), and a stencil of +-reach cells per dimension contains every
-// point the predicate can accept.
+// always within eps under that predicate (the label pass and the symmetric
+// count take a cell's own points without tests), and a stencil of +-reach
+// cells per dimension contains every point the predicate can accept.
 #include <cfloat>
 #include <cmath>
 
@@ -871,6 +871,279 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
     if (tests && lane == 0) tests[t] = issued * tile.w;
 }
 
+// ------------------------------------------------------ symmetric count --
+// Grids without dense cells test every pair of distinct cells once.  Cells
+// are numbered in key order, (c2, c1, c0)-major, so a query's candidates of
+// higher cell index are exactly those of the stencil rows after the centre
+// row and of the centre row's cells after the query's own cell.  A query is
+// tested against those only, and each tested candidate receives the number
+// of queries that accepted it (a popcount of the wave's compare mask).  The
+// pair's decision is the same from either side: (a-b)^2 and (b-a)^2 are the
+// same float.  A query's own cell is counted without tests: the cell side
+// sqrt(e2/d)(1-2^-18) puts every pair of one cell within eps under the
+// float32 predicate (self included, as data_process.cpp:443-447 counts it).
+// Every count is an atomic add into nn_s, which starts at zero.
+
+// a | b << 16 for wave-uniform a, b < 2^16 in one scalar instruction
+__device__ __forceinline__ uint32_t pack16(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("s_pack_ll_b32_b16 %0, %1, %2" : "=s"(r) : "s"(a), "s"(b));
+    return r;
+}
+
+// Candidates [j, j+U) of a wave-uniform range against every lane's query
+// (lanes without a query hold +inf, which accepts nothing); returns the
+// lane's count and puts each candidate's count (a popcount of the compare
+// mask, <= 64) into lane j+u-jb of cacc.  The U = 8 counts are packed a byte
+// each into one wave-uniform 64-bit word, candidate u at byte bpos(u) =
+// (u & 4) | (u & 1) << 1 | (u & 2) >> 1 (pairs are joined at 16 bits, then
+// pairs of pairs at 8); j - jb is a multiple of 8 and lane l unpacks byte
+// bpos(l & 7) (shift `sh`).  MASKED: a lane accepts only candidates at or
+// after my_end (the end of its own cell); the others meet a threshold of -1.
+template <int D, int U, bool MASKED>
+__device__ __forceinline__ int32_t count_block_sym(const float (&qx)[D], const float *__restrict__ soa, int64_t npad,
+                                                   int j, float e2, int my_end, int jb, int lane, uint32_t sh,
+                                                   int32_t &cacc) {
+    float c[D][U];
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+#pragma unroll
+        for (int u = 0; u < U; ++u) c[k][u] = soa[k * npad + j + u];
+    int32_t cnt = 0;
+    uint32_t bc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        float len = 0.0f;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const float df = qx[k] - c[k][u];
+            len = __fadd_rn(len, __fmul_rn(df, df));
+        }
+        const float lim = MASKED ? (j + u >= my_end ? e2 : -1.0f) : e2;
+        const bool ok = len <= lim;
+        cnt += ok ? 1 : 0;
+        bc[u] = (uint32_t)__builtin_popcountll(__ballot(ok));
+    }
+    const int base = j - jb;
+    if constexpr (U == 8) {
+        const uint32_t w0 = pack16(bc[0], bc[1]) | pack16(bc[2], bc[3]) << 8;
+        const uint32_t w1 = pack16(bc[4], bc[5]) | pack16(bc[6], bc[7]) << 8;
+        const uint64_t pk = (uint64_t)w1 << 32 | w0;
+        const int32_t v = (int32_t)((pk >> sh) & 0xffu);
+        cacc = (lane >> 3) == (base >> 3) ? v : cacc;
+    } else {
+        cacc = lane == base ? (int32_t)bc[0] : cacc;
+    }
+    return cnt;
+}
+
+// Candidates [j0, end) in chunks of 64: the chunk's candidate counts are one
+// coalesced atomic add.
+template <int D, bool MASKED>
+__device__ __forceinline__ int32_t count_run_sym(const float (&qx)[D], const float *__restrict__ soa, int64_t npad,
+                                                 int j0, int end, float e2, int my_end, int lane,
+                                                 int32_t *__restrict__ nn_s) {
+    int32_t cnt = 0;
+    const uint32_t sh = 8u * ((lane & 4) | (lane & 1) << 1 | (lane & 2) >> 1);
+    for (int jb = j0; jb < end; jb += 64) {
+        const int je = end - jb < 64 ? end : jb + 64;
+        int32_t cacc = 0;
+        int j = jb;
+        for (; j + 8 <= je; j += 8) cnt += count_block_sym<D, 8, MASKED>(qx, soa, npad, j, e2, my_end, jb, lane, sh, cacc);
+        for (; j < je; ++j) cnt += count_block_sym<D, 1, MASKED>(qx, soa, npad, j, e2, my_end, jb, lane, sh, cacc);
+        if (lane < je - jb && cacc != 0) atomicAdd(nn_s + jb + lane, cacc);
+    }
+    return cnt;
+}
+
+// One wave per count tile, as k_count, over the higher-index candidates only.
+// Tiles of more than 16 queries walk the windows wave-uniformly (scalar
+// candidate loads, one query per lane; cells of 64+ points by the parts in
+// reach of a tile box no wider than a cell).  Smaller tiles put one candidate
+// on each lane and loop over their queries: the candidate's count stays in
+// its lane and each query's count is a popcount of the compare mask.
+template <int D>
+__global__ __launch_bounds__(256) void k_count_sym(const int4 *__restrict__ ctiles, const int32_t *__restrict__ ntp,
+                                                   const int32_t *__restrict__ cell_start,
+                                                   const int32_t *__restrict__ cell_of,
+                                                   const uint64_t *__restrict__ cell_key,
+                                                   const InstGrid *__restrict__ ig, int lin_bits,
+                                                   const int2 *__restrict__ ranges, int rows_per_cell,
+                                                   const int32_t *__restrict__ cell_c, const int32_t *__restrict__ ncp,
+                                                   const float *__restrict__ soa, int64_t npad,
+                                                   int32_t *__restrict__ nn_s, int64_t *__restrict__ tests,
+                                                   const int32_t *__restrict__ part_off) {
+    const int64_t nc = *ncp;
+    const int64_t n_tiles = *ntp;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+    if (t >= n_tiles) return;
+    const int lane = threadIdx.x & 63;
+    const int4 tile = ctiles[t];
+    const int cf = tile.x, cl = tile.y;
+    const bool valid = lane < tile.w;
+    const int q = tile.z + (valid ? lane : 0);
+    float qx[D], lo[D], hi[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        qx[k] = soa[k * npad + q];
+        lo[k] = wave_min(qx[k]);
+        hi[k] = wave_max(qx[k]);
+    }
+    const int my_cell = cell_of[q];
+    const int my_beg = cell_start[my_cell], my_end = cell_start[my_cell + 1];
+    const InstGrid &g = ig[cell_key[cf] >> lin_bits];
+    const float e2 = g.e2;
+    int2 w = tile_window<D>(lane, rows_per_cell, cf, cl, lo, hi, g, cell_c[nc + cf], cell_c[2 * nc + cf], ranges,
+                            cell_c, nc);
+    // rows before the centre row hold lower cells only; the centre row keeps
+    // the cells after the tile's last cell
+    const int centre = rows_per_cell >> 1;
+    if (lane < centre) w = make_int2(0, 0);
+    if (lane == centre && w.x < cl + 1) w.x = cl + 1;
+    if (w.x >= w.y) w = make_int2(0, 0);
+    int ps = w.x < w.y ? cell_start[w.x] : 0, pe = w.x < w.y ? cell_start[w.y] : 0;
+    // the tile's own cells after its first: tested per lane (candidates at
+    // or after the end of the lane's own cell)
+    const int m_beg = cell_start[cf + 1], m_end = cl > cf ? cell_start[cl + 1] : m_beg;
+    int32_t cnt = 0;
+    int64_t issued = 0;
+    if (tile.w > 16) {
+        float qw[D];  // lanes without a query: +inf
+#pragma unroll
+        for (int k = 0; k < D; ++k) qw[k] = valid ? qx[k] : __builtin_inff();
+        issued += m_end - m_beg;
+        cnt += count_run_sym<D, true>(qw, soa, npad, m_beg, m_end, e2, my_end, lane, nn_s);
+        uint64_t rows = __ballot(pe > ps);
+        const double r2o = search_r2(e2);
+        bool small_box = part_off != nullptr;
+#pragma unroll
+        for (int k = 0; k < D; ++k) small_box = small_box && (double)(hi[k] - lo[k]) * g.inv_s <= 1.0;
+        while (rows) {
+            const int slot = __builtin_ctzll(rows);
+            rows &= rows - 1;
+            if (small_box) {
+                const int a = __builtin_amdgcn_readlane(w.x, slot), e = __builtin_amdgcn_readlane(w.y, slot);
+                // the row's cells share dimensions 1..D-1: their half gaps once
+                const double s_ = 1.0 / g.inv_s, h = 0.5 * s_, slack = s_ * 9.5367431640625e-07;
+                double gy[2] = {0.0, 0.0}, gz[2] = {0.0, 0.0};
+#pragma unroll
+                for (int k = 1; k < D; ++k) {
+                    const double X = g.origin[k] + (double)cell_c[k * nc + a] * s_;
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        const double a0 = X + (double)b * h - slack, a1 = a0 + h + 2.0 * slack;
+                        const double gp = fmax(0.0, fmax(a0 - (double)hi[k], (double)lo[k] - a1));
+                        (k == 1 ? gy : gz)[b] = gp * gp;
+                    }
+                }
+                int run = cell_start[a];  // pending run of whole cells
+                for (int B = a; B <= e; ++B) {
+                    const int32_t b0 = cell_start[B];
+                    // cells with few points are taken whole (merged into the run)
+                    if (B < e && cell_start[B + 1] - b0 < kPartMin) continue;
+                    issued += b0 - run;
+                    cnt += count_run_sym<D, false>(qw, soa, npad, run, b0, e2, my_end, lane, nn_s);
+                    if (B == e) break;
+                    const double X = g.origin[0] + (double)cell_c[B] * s_;
+                    double gx[2];
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        const double a0 = X + (double)b * h - slack, a1 = a0 + h + 2.0 * slack;
+                        const double gp = fmax(0.0, fmax(a0 - (double)hi[0], (double)lo[0] - a1));
+                        gx[b] = gp * gp;
+                    }
+                    uint32_t mask = 0;
+#pragma unroll
+                    for (int o = 0; o < (1 << D); ++o) {
+                        double sum = gx[(o >> part_bit<D>(0)) & 1];
+                        if (D >= 2) sum += gy[(o >> part_bit<D>(1)) & 1];
+                        if (D >= 3) sum += gz[(o >> part_bit<D>(2)) & 1];
+                        if (sum <= r2o) mask |= 1u << o;
+                    }
+                    const int32_t *po = part_off + (int64_t)B * 8;
+                    const int32_t b_end = cell_start[B + 1];
+                    while (mask) {
+                        const int o1 = __builtin_ctz(mask);
+                        const uint32_t above = ~((1u << o1) - 1u);
+                        const int o2 = (~mask & above) ? __builtin_ctz(~mask & above) : (1 << D);
+                        mask &= o2 < 32 ? ~((1u << o2) - 1u) : 0u;
+                        const int jj = po[o1];
+                        const int end = o2 < (1 << D) ? po[o2] : b_end;
+                        issued += end - jj;
+                        cnt += count_run_sym<D, false>(qw, soa, npad, jj, end, e2, my_end, lane, nn_s);
+                    }
+                    run = b_end;
+                }
+                continue;
+            }
+            const int end = __builtin_amdgcn_readlane(pe, slot);
+            const int j = __builtin_amdgcn_readlane(ps, slot);
+            issued += end - j;
+            cnt += count_run_sym<D, false>(qw, soa, npad, j, end, e2, my_end, lane, nn_s);
+        }
+        issued *= tile.w;
+    } else {
+        // one candidate per lane, the tile's queries in turn; the tile's own
+        // cells after its first are lane 63's window (lanes >= rows_per_cell
+        // have none)
+        if (lane == 63) {
+            ps = m_beg;
+            pe = m_end;
+        }
+        const int len = pe - ps;
+        int incl = len;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t2 = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t2;
+        }
+        const int pre = incl - len;
+        const int total = __builtin_amdgcn_readlane(incl, 63);
+        int32_t qcnt[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) qcnt[i] = 0;
+        for (int cb = 0; cb < total; cb += 64) {
+            const int gi = cb + lane;
+            const bool okc = gi < total;
+            const int gc = okc ? gi : total - 1;  // all lanes take part in the shuffles
+            int lo_w = 0, hi_w = 63;  // last lane whose window starts at or before gc
+#pragma unroll
+            for (int it = 0; it < 6; ++it) {
+                const int mid = (lo_w + hi_w + 1) >> 1;
+                if (__shfl(pre, mid, 64) <= gc) lo_w = mid;
+                else hi_w = mid - 1;
+            }
+            const int j = __shfl(ps, lo_w, 64) + gc - __shfl(pre, lo_w, 64);
+            float c[D];
+#pragma unroll
+            for (int k = 0; k < D; ++k) c[k] = soa[k * npad + j];
+            int32_t ccount = 0;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                if (i < tile.w) {
+                    float l2 = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < D; ++k) {
+                        const float df = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qx[k]), i)) - c[k];
+                        l2 = __fadd_rn(l2, __fmul_rn(df, df));
+                    }
+                    const bool ok = okc && l2 <= e2 && j >= __builtin_amdgcn_readlane(my_end, i);
+                    ccount += ok ? 1 : 0;
+                    qcnt[i] += __builtin_popcountll(__ballot(ok));
+                }
+            }
+            if (okc && ccount != 0) atomicAdd(nn_s + j, ccount);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (lane == i) cnt = qcnt[i];
+        issued = (int64_t)total * tile.w;
+    }
+    if (valid) atomicAdd(nn_s + q, cnt + (my_end - my_beg));
+    if (tests && lane == 0) tests[t] = issued;
+}
+
 // Sum of int64 values (pair-test statistics): one atomic per block.
 __global__ __launch_bounds__(256) void k_sum_i64(const int64_t *__restrict__ v, int64_t n,
                                                  unsigned long long *__restrict__ out) {
@@ -1092,12 +1365,23 @@ static void grid_classify_d(Ctx &c) {
         c.tile_tests.ensure(nct * 8);
         fill_u64(reinterpret_cast<uint64_t *>(d_tests), 0, 1, s, c.prof);
     }
-    {
+    if (dv.id) {
+        // dense 2-D cells: every tile counts its own queries against all candidates
+        if constexpr (D == 2) {
+            Launch L(c.prof, s, "count");
+            k_count<D><<<(unsigned)ceil_div(nct, 4), 256, 0, s>>>(
+                c.ctiles.as<int4>(), ntp, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig, c.lin_bits,
+                c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), ncp, c.soa.as<float>(), npad, c.nn_s.as<int32_t>(),
+                stats ? c.tile_tests.as<int64_t>() : nullptr, dv, part_off);
+        }
+    } else {
+        // each pair of distinct cells tested once, counts added to both points
+        fill_i32(c.nn_s.as<int32_t>(), 0, n, s, c.prof);
         Launch L(c.prof, s, "count");
-        k_count<D><<<(unsigned)ceil_div(nct, 4), 256, 0, s>>>(
-            c.ctiles.as<int4>(), ntp, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig, c.lin_bits,
-            c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), ncp, c.soa.as<float>(), npad, c.nn_s.as<int32_t>(),
-            stats ? c.tile_tests.as<int64_t>() : nullptr, dv, part_off);
+        k_count_sym<D><<<(unsigned)ceil_div(nct, 4), 256, 0, s>>>(
+            c.ctiles.as<int4>(), ntp, c.cell_start.as<int32_t>(), c.cell_of.as<int32_t>(), c.cell_key.as<uint64_t>(),
+            ig, c.lin_bits, c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), ncp, c.soa.as<float>(), npad,
+            c.nn_s.as<int32_t>(), stats ? c.tile_tests.as<int64_t>() : nullptr, part_off);
     }
     if (stats) {
         Launch L(c.prof, s, "stats");
