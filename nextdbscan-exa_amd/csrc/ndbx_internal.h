@@ -161,7 +161,7 @@ struct Ctx {
     int64_t graph_replays = 0, graph_reruns = 0;
 
     // brute / mfma path state
-    DevBuf b_parent, b_rootmin;
+    DevBuf b_parent, b_rootmin, pflag;
 
     // audit (mfma)
     int64_t audit_band = 0, audit_fp64_flips = 0, audit_filter_flips = 0;

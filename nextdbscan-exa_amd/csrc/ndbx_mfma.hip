@@ -154,11 +154,12 @@ __global__ __launch_bounds__(256) void k_cell_radius(const float *__restrict__ x
     }
 }
 
-// Candidate cell pairs: |centre_C - centre_C'| <= R_C + R_C' + r (+ slack).
-// Pass 0 counts per cell, pass 1 writes (in no particular order: every use is
-// order-independent).
-// Pass 1 also adds |C| * (points of C's candidate cells) to *pts: the point
-// pairs the filter evaluates (one atomic per cell).
+// Candidate cell pairs: |centre_C - centre_C'| <= R_C + R_C' + r (+ slack),
+// a symmetric relation (the same float64 sums from either side).  Pass 0
+// counts per cell; pass 1 writes each cell's list in ascending cell order
+// (a block scan per 256 candidates), so the entry of C in the list of C' can
+// be found by binary search.  Pass 1 also adds |C| * (points of C's
+// candidate cells) to *pts: the point pairs the filter covers.
 __global__ __launch_bounds__(256) void k_cell_pairs(const float *__restrict__ cent, const uint64_t *__restrict__ cell_key,
                                                     const double *__restrict__ rad, int64_t nc, int d, double r,
                                                     const int32_t *__restrict__ off, int32_t *__restrict__ cnt_or_list,
@@ -167,34 +168,41 @@ __global__ __launch_bounds__(256) void k_cell_pairs(const float *__restrict__ ce
     const int64_t c = (int64_t)blockIdx.x;
     if (c >= nc) return;
     const float *a = cent + (int64_t)cell_key[c] * d;
-    __shared__ int32_t cnt;
+    __shared__ int32_t wsum[4];
     __shared__ unsigned long long cand_pts;
-    if (threadIdx.x == 0) {
-        cnt = 0;
-        cand_pts = 0;
-    }
-    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) cand_pts = 0;
+    int32_t base = 0;  // entries written so far (pass 1) or counted (pass 0)
     unsigned long long mine = 0;
-    for (int64_t o = threadIdx.x; o < nc; o += blockDim.x) {
-        const float *b = cent + (int64_t)cell_key[o] * d;
-        double s = 0.0;
-        for (int k = 0; k < d; ++k) {
-            const double df = (double)a[k] - (double)b[k];
-            s += df * df;
-        }
-        const double lim = rad[c] + rad[o] + r;
-        if (s <= lim * lim * (1.0 + 1e-9)) {
-            if (pass == 0) {
-                atomicAdd(&cnt, 1);
-            } else {
-                cnt_or_list[off[c] + atomicAdd(&cnt, 1)] = (int32_t)o;
-                mine += (unsigned long long)(cell_start[o + 1] - cell_start[o]);
+    for (int64_t o0 = 0; o0 < nc; o0 += blockDim.x) {
+        const int64_t o = o0 + threadIdx.x;
+        bool in = false;
+        if (o < nc) {
+            const float *b = cent + (int64_t)cell_key[o] * d;
+            double s = 0.0;
+            for (int k = 0; k < d; ++k) {
+                const double df = (double)a[k] - (double)b[k];
+                s += df * df;
             }
+            const double lim = rad[c] + rad[o] + r;
+            in = s <= lim * lim * (1.0 + 1e-9);
         }
+        const uint64_t m = __ballot(in);
+        __syncthreads();  // wsum of the previous round has been read
+        if (lane == 0) wsum[wave] = __popcll(m);
+        __syncthreads();
+        int32_t before = base;
+        for (int w = 0; w < wave; ++w) before += wsum[w];
+        const int32_t pos = before + (int32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (in && pass == 1) {
+            cnt_or_list[off[c] + pos] = (int32_t)o;
+            mine += (unsigned long long)(cell_start[o + 1] - cell_start[o]);
+        }
+        base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
     }
     if (pass == 1 && mine) atomicAdd(&cand_pts, mine);
     __syncthreads();
-    if (pass == 0 && threadIdx.x == 0) cnt_or_list[c] = cnt;
+    if (pass == 0 && threadIdx.x == 0) cnt_or_list[c] = base;
     if (pass == 1 && threadIdx.x == 0 && cand_pts)
         atomicAdd(pts, cand_pts * (unsigned long long)(cell_start[c + 1] - cell_start[c]));
 }
@@ -271,6 +279,7 @@ struct MfmaArgs {
     int32_t *parent;                  // pass 1
     const int32_t *label_s;           // pass 2
     int32_t *best_s;                  // pass 2
+    uint8_t *pflag;                   // per candidate entry: the pair holds a pair within eps
     unsigned long long *audit;        // [band pairs, fp64 flips, filter flips], pass 3
     float *audit_err;                 // max |est - fp64 len|, pass 3
     int audit_stride;
@@ -293,9 +302,20 @@ __global__ __launch_bounds__(256) void k_group_count(const int32_t *__restrict__
     if (c < nc) cnt[c] = (toff[c + 1] - toff[c] + 3) / 4;
 }
 
-// PASS 0: neighbour counts.  PASS 1: unite core pairs (col > row).  PASS 2:
-// smallest core cluster id within eps for non-core rows.  PASS 3: audit of
-// every estimate of every audit_stride-th group against float32 and float64.
+// Each unordered pair of candidate cells is computed once: cell c takes
+// candidate o when o == c, or when (c < o) equals (c + o even) -- about half
+// of every cell's candidates, whatever the cell order.  A pair (c, o != c)
+// adds its decisions to the rows' counts and to the columns' counts; the
+// diagonal pair (c, c) adds to the rows only (every row of c meets every
+// column of c, itself included).
+__device__ __forceinline__ bool mf_takes(int c, int o) { return o == c || ((c < o) == (((c + o) & 1) == 0)); }
+
+// PASS 0: neighbour counts.  PASS 1: unite core pairs, over the cell pairs
+// this cell takes whose count pass found a pair within eps (col > row in
+// the cell's own pair).  PASS 2: smallest core cluster id within eps for
+// non-core rows, over the candidate cells whose pair was flagged (the entry
+// of the side that takes it).  PASS 3: audit of every estimate of every
+// audit_stride-th group against float32 and float64.
 // Rows and columns that take no part are masked by an infinite norm, so the
 // estimate never passes; the few estimates inside the guard band are decided
 // afterwards with the exact predicate, under a ballot.
@@ -373,18 +393,33 @@ __global__ __launch_bounds__(256) void k_mfma_pass(MfmaArgs A) {
     constexpr int UPT = (UNITS + 255) / 256;  // units per thread
     const int c_beg = A.cand_off[cell], c_end = A.cand_off[cell + 1];
     int ci = c_beg, cb = 0, ce = 0, cs = 0;
+    // the candidate cells a pass visits (an empty range for the others)
+    auto wanted = [&](int ci_, int o) -> bool {
+        if (PASS == 0 || PASS == 3) return true;
+        if (mf_takes(cell, o)) return A.pflag[ci_] != 0;
+        if (PASS == 1) return false;
+        int lo_ = A.cand_off[o], hi_ = A.cand_off[o + 1];  // entry of `cell` in o's list
+        const int end_ = hi_;
+        while (lo_ < hi_) {
+            const int mid = (lo_ + hi_) >> 1;
+            if (A.cand[mid] < cell) lo_ = mid + 1;
+            else hi_ = mid;
+        }
+        return lo_ < end_ && A.cand[lo_] == cell && A.pflag[lo_] != 0;
+    };
     auto load_cell = [&](int ci_, int &cb_, int &cs_, int &ce_) {
         const int cc_ = A.cand[ci_];
         cs_ = A.cell_start[cc_];
         ce_ = A.cell_start[cc_ + 1];
-        cb_ = cs_;
+        cb_ = wanted(ci_, cc_) ? cs_ : ce_;
     };
     // move (ci, cb) to the first block at or after it that has work; ci ==
-    // c_end when none is left.  PASS 1 jumps over blocks that lie entirely
-    // below every row of the group.
+    // c_end when none is left.  PASS 1 jumps over blocks of the cell's own
+    // pair that lie entirely below every row of the group.
     auto settle = [&](int &ci_, int &cb_, int &cs_, int &ce_) {
         while (ci_ < c_end) {
-            if (PASS == 1 && cb_ + 31 < min_row_point) cb_ += 32 * ((min_row_point - cb_) / 32);
+            if (PASS == 1 && cb_ + 31 < min_row_point && A.cand[ci_] == cell)
+                cb_ += 32 * ((min_row_point - cb_) / 32);
             if (cb_ < ce_) return;
             ++ci_;
             if (ci_ < c_end) load_cell(ci_, cb_, cs_, ce_);
@@ -563,7 +598,7 @@ __global__ __launch_bounds__(256) void k_mfma_pass(MfmaArgs A) {
             if (PASS == 1 && in_m) {
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    if (!((in_m >> r) & 1) || bcol <= rp_r[r] || root_r[r] == aux) continue;
+                    if (!((in_m >> r) & 1) || (cc == cell && bcol <= rp_r[r]) || root_r[r] == aux) continue;
                     mf_unite(A.parent, rp_r[r], bcol);
                     root_r[r] = mf_find(A.parent, rp_r[r]);
                 }
@@ -600,6 +635,218 @@ __global__ __launch_bounds__(256) void k_mfma_pass(MfmaArgs A) {
                 if (PASS == 0) A.nn_s[rp_r[r]] = acc_i[r];
                 if (PASS == 2) A.best_s[rp_r[r]] = acc_i[r];
             }
+        }
+    }
+}
+
+// ------------------------------------------------------ symmetric count --
+// Groups are taken in chunks of `chunk` consecutive groups (a few cells),
+// chunk q on XCD q % 8 (workgroups go to XCDs round-robin by index), so the
+// groups of one cell run on one XCD at about the same time and read their
+// candidate blocks from its L2.
+__device__ __forceinline__ int64_t mf_group_of(int64_t b, int chunk) {
+    const int64_t x = b & 7, i = b >> 3;
+    return ((i / chunk) * 8 + x) * chunk + (i % chunk);
+}
+
+// PASS 0 as k_mfma_pass, over the cell pairs this cell takes.  nn_s starts at
+// zero; row and column counts are atomic adds.  Column counts of a block are
+// summed over the workgroup's waves in LDS and added with one atomic per
+// column after the block's barrier.
+template <int KP>
+__global__ __launch_bounds__(256, KP >= 128 ? 2 : 3) void k_mf_count(MfmaArgs A, int chunk) {
+    constexpr int KS = KP / 16;
+    constexpr int UNITS = 32 * (KP / 8);
+    constexpr int ROWU = KP / 8 + 1;
+    constexpr int UPT = (UNITS + 255) / 256;
+    __shared__ bf16x8_t sb_hi[2][32 * ROWU], sb_lo[2][32 * ROWU];
+    __shared__ float sb_na[2][32];
+    __shared__ int32_t sb_col[2][32];
+    __shared__ int32_t sb_cell[2], sb_cb[2], sb_ci[2];
+    const int64_t gidx = mf_group_of(blockIdx.x, chunk);
+    if (gidx >= A.n_groups) return;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int4 grp = A.groups[gidx];
+    const bool wave_on = wave < grp.z;
+    const int4 tile = A.tiles[grp.y + (wave_on ? wave : 0)];
+    const int lane = threadIdx.x & 63;
+    const int half = lane >> 5, col = lane & 31;
+    const int cell = grp.x, row0 = tile.y, nrows = wave_on ? tile.z : 0;
+    const bf16x8_t *hi8 = reinterpret_cast<const bf16x8_t *>(A.hi);
+    const bf16x8_t *lo8 = reinterpret_cast<const bf16x8_t *>(A.lo);
+    const float INF = __builtin_inff();
+    // A operand: row `col` of the tile, k elements [16 ks + 8 half, +8)
+    const bool arow_ok = col < nrows;
+    const int64_t arow = row0 + (arow_ok ? col : 0);
+    bf16x8_t ahi[KS], alo[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+        ahi[ks] = arow_ok ? hi8[arow * (KP / 8) + ks * 2 + half] : bf16x8_t{};
+        alo[ks] = arow_ok ? lo8[arow * (KP / 8) + ks * 2 + half] : bf16x8_t{};
+    }
+    // accumulator rows: r -> tile row 8 (r/4) + 4 half + (r%4); rows past the
+    // tile have mid = +inf, so dv = -inf: never counted, never in the band
+    float mid_r[16];
+    int32_t rcnt[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int tr = 8 * (r >> 2) + 4 * half + (r & 3);
+        mid_r[r] = tr < nrows ? 0.5f * (A.na[row0 + tr] - A.e2) : INF;
+        rcnt[r] = 0;
+    }
+    float rmax_row = arow_ok ? A.rho[arow] : 0.0f;
+    for (int o = 32; o >= 1; o >>= 1) rmax_row = fmaxf(rmax_row, __shfl_xor(rmax_row, o, 64));
+    rmax_row = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(rmax_row)));
+
+    // block sequence: the 32-column blocks of the cells this cell takes
+    const int c_beg = A.cand_off[cell], c_end = A.cand_off[cell + 1];
+    int ci = c_beg, cb = 0, ce = 0, co = -1;
+    auto settle = [&]() {  // first block at or after (ci, cb) of a taken cell
+        while (ci < c_end) {
+            if (co >= 0 && cb < ce) return;
+            if (co >= 0) ++ci;
+            co = -1;
+            if (ci >= c_end) return;
+            const int o = A.cand[ci];
+            if (!mf_takes(cell, o)) {
+                ++ci;
+                continue;
+            }
+            co = o;
+            cb = A.cell_start[o];
+            ce = A.cell_start[o + 1];
+        }
+    };
+    bf16x8_t pf_hi[UPT], pf_lo[UPT];
+    float pf_na = INF;
+    auto fetch = [&]() {
+#pragma unroll
+        for (int q = 0; q < UPT; ++q) {
+            const int u = threadIdx.x + q * 256;
+            const int bc = u / (KP / 8), kk = u - bc * (KP / 8);
+            const bool ok = u < UNITS && cb + bc < ce;
+            const int64_t src = (int64_t)(ok ? cb + bc : cb) * (KP / 8) + kk;
+            pf_hi[q] = ok ? hi8[src] : bf16x8_t{};
+            pf_lo[q] = ok ? lo8[src] : bf16x8_t{};
+        }
+        if (threadIdx.x < 32) pf_na = cb + (int)threadIdx.x < ce ? A.na[cb + threadIdx.x] : INF;
+    };
+    auto publish = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < UPT; ++q) {
+            const int u = threadIdx.x + q * 256;
+            if (u < UNITS) {
+                const int bc = u / (KP / 8), kk = u - bc * (KP / 8);
+                sb_hi[buf][bc * ROWU + kk] = pf_hi[q];
+                sb_lo[buf][bc * ROWU + kk] = pf_lo[q];
+            }
+        }
+        if (threadIdx.x < 32) {
+            sb_na[buf][threadIdx.x] = pf_na;
+            sb_col[buf][threadIdx.x] = 0;
+        }
+        if (threadIdx.x == 0) {
+            sb_cell[buf] = co;
+            sb_cb[buf] = cb;
+            sb_ci[buf] = ci;
+        }
+    };
+    settle();
+    if (ci >= c_end) return;  // cannot happen: a cell always takes itself
+    fetch();
+    publish(0);
+    __syncthreads();
+    int buf = 0, cur_o = -1;
+    float hb = 0.0f;
+    for (;;) {
+        // the next block's loads overlap this block's products
+        cb += 32;
+        settle();
+        const bool more = ci < c_end;
+        if (more) fetch();
+        const int o = sb_cell[buf], bcb = sb_cb[buf];
+        const bool diag = o == cell;
+        if (wave_on) {
+            if (o != cur_o) {
+                cur_o = o;
+                const double R = fmax((double)rmax_row, (double)A.cell_rmax[o]);
+                hb = (float)(A.b1 * R * R + A.b2 * R + A.b3);  // half the band
+            }
+            const float nb = sb_na[buf][col];
+            f32x16_t acc;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[r] = -0.5f * nb;
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const bf16x8_t bhi = sb_hi[buf][col * ROWU + ks * 2 + half];
+                const bf16x8_t blo = sb_lo[buf][col * ROWU + ks * 2 + half];
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[ks], bhi, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[ks], blo, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[ks], bhi, acc, 0, 0, 0);
+            }
+            // dv = acc - (|a|^2 - e2)/2 = (e2 - est)/2: dv >= hb is a neighbour,
+            // dv < -hb is not, the rest (the guard band) the exact predicate decides
+            int32_t ccnt = 0;
+            float amin = INF;
+#pragma unroll
+            for (int r = 0; r < 16; r += 2) {
+                const float dv0 = acc[r] - mid_r[r], dv1 = acc[r + 1] - mid_r[r + 1];
+                const int32_t in0 = dv0 >= hb ? 1 : 0, in1 = dv1 >= hb ? 1 : 0;
+                rcnt[r] += in0;
+                rcnt[r + 1] += in1;
+                ccnt += in0 + in1;
+                amin = fminf(amin, fminf(fabsf(dv0), fabsf(dv1)));
+            }
+            if (__ballot(amin <= hb) != 0) {
+                // band elements (-hb <= dv < hb) one at a time per lane
+                uint32_t bm = 0, hit = 0;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const float dv = acc[r] - mid_r[r];
+                    bm |= (uint32_t)(dv < hb && dv >= -hb) << r;
+                }
+                const float *pc = A.xs + (int64_t)(bcb + col) * A.d;
+                while (bm) {
+                    const int r = __builtin_ctz(bm);
+                    bm &= bm - 1;
+                    const int tr = 8 * (r >> 2) + 4 * half + (r & 3);
+                    if (exact_pred(A.xs + (int64_t)(row0 + tr) * A.d, pc, A.d, A.e2)) hit |= 1u << r;
+                }
+                ccnt += __builtin_popcount(hit);
+#pragma unroll
+                for (int r = 0; r < 16; ++r) rcnt[r] += (hit >> r) & 1;
+            }
+            // the pair holds a pair within eps: the label passes visit it
+            if (__ballot(ccnt != 0) != 0 && lane == 0) A.pflag[sb_ci[buf]] = 1;
+            if (!diag) {
+                ccnt += __shfl_xor(ccnt, 32, 64);
+                if (half == 0 && ccnt != 0) atomicAdd(&sb_col[buf][col], ccnt);
+            }
+        }
+        if (more) publish(buf ^ 1);
+        __syncthreads();
+        // column counts of the block just computed (its stage is rewritten
+        // only after the next barrier)
+        if (!diag && threadIdx.x < 32) {
+            const int32_t v = sb_col[buf][threadIdx.x];
+            if (v != 0) atomicAdd(A.nn_s + bcb + threadIdx.x, v);
+        }
+        if (!more) break;
+        buf ^= 1;
+    }
+    if (!wave_on) return;
+    // each row over the 32 lanes (columns) of its half
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        int32_t v = rcnt[r];
+        for (int o = 16; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+        rcnt[r] = v;
+    }
+    if (col == 0) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int tr = 8 * (r >> 2) + 4 * half + (r & 3);
+            if (tr < nrows && rcnt[r] != 0) atomicAdd(A.nn_s + row0 + tr, rcnt[r]);
         }
     }
 }
@@ -719,6 +966,21 @@ static void launch_pass(Ctx &c, const MfmaArgs &a, int kp, const char *name) {
     }
 }
 
+// Symmetric count pass: groups in XCD chunks of kCountChunk.
+constexpr int kCountChunk = 16;
+static void launch_count(Ctx &c, const MfmaArgs &a, int kp) {
+    if (a.n_groups == 0) return;
+    const int64_t per = 8 * kCountChunk;
+    const unsigned g = (unsigned)(ceil_div(a.n_groups, per) * per);
+    Launch L(c.prof, c.stream, "mf_count");
+    switch (kp) {
+        case 16: k_mf_count<16><<<g, 256, 0, c.stream>>>(a, kCountChunk); break;
+        case 32: k_mf_count<32><<<g, 256, 0, c.stream>>>(a, kCountChunk); break;
+        case 64: k_mf_count<64><<<g, 256, 0, c.stream>>>(a, kCountChunk); break;
+        default: k_mf_count<128><<<g, 256, 0, c.stream>>>(a, kCountChunk); break;
+    }
+}
+
 static MfmaArgs mfma_args(Ctx &c) {
     MfmaArgs a{};
     a.tiles = c.ctiles.as<int4>();
@@ -734,6 +996,7 @@ static MfmaArgs mfma_args(Ctx &c) {
     a.rho = c.soa_core.as<float>() + c.n;
     a.cell_rmax = c.cell_c.as<float>();
     a.xs = c.soa.as<float>();
+    a.pflag = c.pflag.as<uint8_t>();
     a.n = c.n;
     a.d = c.d;
     const float e = c.h_eps[0];
@@ -852,6 +1115,8 @@ void mfma_classify(Ctx &c) {
     NDBX_HIP(hipMemcpyAsync(&h_pairs, c.tile_off.as<int32_t>() + nc, 4, hipMemcpyDeviceToHost, s));
     c.sync();
     c.ranges.ensure((size_t)h_pairs * 4 + 4);
+    c.pflag.ensure((size_t)h_pairs + 4);
+    fill_i32(c.pflag.as<int32_t>(), 0, ceil_div(h_pairs, 4), s, c.prof);
     unsigned long long *d_pts = reinterpret_cast<unsigned long long *>(c.misc.as<int32_t>() + 620);
     fill_u64(reinterpret_cast<uint64_t *>(d_pts), 0, 1, s, c.prof);
     {
@@ -900,7 +1165,8 @@ void mfma_classify(Ctx &c) {
     c.nn_s.ensure(n * 4);
     MfmaArgs a = mfma_args(c);
     a.nn_s = c.nn_s.as<int32_t>();
-    launch_pass<0>(c, a, kp, "mf_count");
+    fill_i32(a.nn_s, 0, n, s, c.prof);
+    launch_count(c, a, kp);
     c.core_s.ensure(n);
     c.label_s.ensure(n * 4);
     c.parent.ensure(n * 4);
