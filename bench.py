@@ -248,11 +248,13 @@ def main():
         avg_s = total_ms / launches * 1e-3
         ab = algorithmic_bytes(dominant, n, d, stats["cells"])
         if dominant == "mf_count":
-            # distance filter: F = 2d flops per (ordered) point pair of the
-            # candidate cell pairs; the kernel issues 3 bf16 MFMAs per 16-wide
-            # k step (hi*hi, hi*lo, lo*hi) with k padded to kp
+            # distance filter: F = 2d flops (a d-term dot product) per unordered
+            # point pair of the candidate cell pairs, i.e. d per ordered pair
+            # (pair_tests counts ordered pairs); the kernel computes each
+            # unordered pair once with 3 bf16 MFMAs per 16-wide k step (hi*hi,
+            # hi*lo, lo*hi), k padded to kp: 3 kp issued flops per ordered pair
             kp = 16 if d <= 16 else 32 if d <= 32 else 64 if d <= 64 else 128
-            flops = 2.0 * d * float(stats["pair_tests"])
+            flops = 1.0 * d * float(stats["pair_tests"])
             tf = flops / avg_s / 1e12
             roofline = {"bound": "mfma", "kernel": dominant, "achieved": tf, "peak": BF16_DENSE_PEAK_TFLOPS,
                         "unit": "TFLOP/s", "frac": tf / BF16_DENSE_PEAK_TFLOPS,

@@ -27,6 +27,14 @@
 namespace ndbx {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+// A load of read-only data through the constant address space: at a
+// wave-uniform address it is a scalar load, which waits on its own counter
+// instead of behind the vector loads in flight.
+template <class T>
+__device__ __forceinline__ T ld_const(const T *p) {
+    return *(const __attribute__((address_space(4))) T *)(p);
+}
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
 constexpr int kCellTarget = 1024;  // points per partition cell, on average
@@ -650,9 +658,15 @@ __device__ __forceinline__ int64_t mf_group_of(int64_t b, int chunk) {
 }
 
 // PASS 0 as k_mfma_pass, over the cell pairs this cell takes.  nn_s starts at
-// zero; row and column counts are atomic adds.  Column counts of a block are
-// summed over the workgroup's waves in LDS and added with one atomic per
-// column after the block's barrier.
+// zero; row and column counts are atomic adds.  Column counts are summed in
+// an LDS window of kColWin consecutive columns of one candidate cell (all
+// waves, all blocks of the window) and added to nn_s, one atomic per
+// column, when the block sequence leaves the window; so is the pair flag.
+// The blocks themselves write nothing to global memory (a store or atomic
+// there would hold up the next block's loads, which wait on the same
+// counter).
+constexpr int kColWin = 2048;
+
 template <int KP>
 __global__ __launch_bounds__(256, KP >= 128 ? 2 : 3) void k_mf_count(MfmaArgs A, int chunk) {
     constexpr int KS = KP / 16;
@@ -661,8 +675,9 @@ __global__ __launch_bounds__(256, KP >= 128 ? 2 : 3) void k_mf_count(MfmaArgs A,
     constexpr int UPT = (UNITS + 255) / 256;
     __shared__ bf16x8_t sb_hi[2][32 * ROWU], sb_lo[2][32 * ROWU];
     __shared__ float sb_na[2][32];
-    __shared__ int32_t sb_col[2][32];
-    __shared__ int32_t sb_cell[2], sb_cb[2], sb_ci[2];
+    __shared__ int32_t sb_cacc[kColWin];  // column counts of the current window
+    __shared__ int32_t sb_cell[2], sb_cb[2], sb_ci[2], sb_cs[2], sb_edge;
+    __shared__ float sb_rmax[2];
     const int64_t gidx = mf_group_of(blockIdx.x, chunk);
     if (gidx >= A.n_groups) return;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -699,37 +714,49 @@ __global__ __launch_bounds__(256, KP >= 128 ? 2 : 3) void k_mf_count(MfmaArgs A,
     rmax_row = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(rmax_row)));
 
     // block sequence: the 32-column blocks of the cells this cell takes
-    const int c_beg = A.cand_off[cell], c_end = A.cand_off[cell + 1];
-    int ci = c_beg, cb = 0, ce = 0, co = -1;
+    const int c_beg = ld_const(A.cand_off + cell), c_end = ld_const(A.cand_off + cell + 1);
+    int ci = c_beg, cb = 0, cs = 0, ce = 0, co = -1;
     auto settle = [&]() {  // first block at or after (ci, cb) of a taken cell
         while (ci < c_end) {
             if (co >= 0 && cb < ce) return;
             if (co >= 0) ++ci;
             co = -1;
             if (ci >= c_end) return;
-            const int o = A.cand[ci];
+            const int o = ld_const(A.cand + ci);
             if (!mf_takes(cell, o)) {
                 ++ci;
                 continue;
             }
             co = o;
-            cb = A.cell_start[o];
-            ce = A.cell_start[o + 1];
+            cs = cb = ld_const(A.cell_start + o);
+            ce = ld_const(A.cell_start + o + 1);
         }
     };
+    // Columns past the cell end keep whatever the clamped loads return (the
+    // rows of the next cell, or the last row): their norm is +inf, so their
+    // accumulator is -inf whatever the products are.
+    // (The norm is selected only when published: a select right after its
+    // load would wait for every load of the fetch.)
     bf16x8_t pf_hi[UPT], pf_lo[UPT];
-    float pf_na = INF;
+    float pf_na = INF, pf_rmax = 0.0f;
+    bool pf_ok = false;
+    const int64_t n_last = A.n - 1;
     auto fetch = [&]() {
 #pragma unroll
         for (int q = 0; q < UPT; ++q) {
             const int u = threadIdx.x + q * 256;
             const int bc = u / (KP / 8), kk = u - bc * (KP / 8);
-            const bool ok = u < UNITS && cb + bc < ce;
-            const int64_t src = (int64_t)(ok ? cb + bc : cb) * (KP / 8) + kk;
-            pf_hi[q] = ok ? hi8[src] : bf16x8_t{};
-            pf_lo[q] = ok ? lo8[src] : bf16x8_t{};
+            const int64_t pt = cb + bc < n_last ? cb + bc : n_last;
+            const int64_t src = pt * (KP / 8) + kk;
+            if (u < UNITS) {
+                pf_hi[q] = hi8[src];
+                pf_lo[q] = lo8[src];
+            }
         }
-        if (threadIdx.x < 32) pf_na = cb + (int)threadIdx.x < ce ? A.na[cb + threadIdx.x] : INF;
+        const int jn = cb + (int)(threadIdx.x & 31);
+        pf_na = A.na[jn < n_last ? jn : n_last];
+        pf_ok = jn < ce;
+        pf_rmax = ld_const(A.cell_rmax + co);
     };
     auto publish = [&](int buf) {
 #pragma unroll
@@ -741,18 +768,19 @@ __global__ __launch_bounds__(256, KP >= 128 ? 2 : 3) void k_mf_count(MfmaArgs A,
                 sb_lo[buf][bc * ROWU + kk] = pf_lo[q];
             }
         }
-        if (threadIdx.x < 32) {
-            sb_na[buf][threadIdx.x] = pf_na;
-            sb_col[buf][threadIdx.x] = 0;
-        }
+        if (threadIdx.x < 32) sb_na[buf][threadIdx.x] = pf_ok ? pf_na : INF;
         if (threadIdx.x == 0) {
             sb_cell[buf] = co;
             sb_cb[buf] = cb;
             sb_ci[buf] = ci;
+            sb_cs[buf] = cs;
+            sb_rmax[buf] = pf_rmax;
         }
     };
     settle();
     if (ci >= c_end) return;  // cannot happen: a cell always takes itself
+    for (int t = threadIdx.x; t < kColWin; t += blockDim.x) sb_cacc[t] = 0;
+    if (threadIdx.x == 0) sb_edge = 0;
     fetch();
     publish(0);
     __syncthreads();
@@ -764,18 +792,26 @@ __global__ __launch_bounds__(256, KP >= 128 ? 2 : 3) void k_mf_count(MfmaArgs A,
         settle();
         const bool more = ci < c_end;
         if (more) fetch();
-        const int o = sb_cell[buf], bcb = sb_cb[buf];
+        // (wave-uniform: the cell radius below is then a scalar load, which
+        // does not wait for the fetch's vector loads)
+        const int o = __builtin_amdgcn_readfirstlane(sb_cell[buf]);
+        const int bcb = __builtin_amdgcn_readfirstlane(sb_cb[buf]);
+        const int bci = __builtin_amdgcn_readfirstlane(sb_ci[buf]);
+        const int bcs = __builtin_amdgcn_readfirstlane(sb_cs[buf]);
         const bool diag = o == cell;
         if (wave_on) {
             if (o != cur_o) {
                 cur_o = o;
-                const double R = fmax((double)rmax_row, (double)A.cell_rmax[o]);
+                const float rmax_col = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(sb_rmax[buf])));
+                const double R = fmax((double)rmax_row, (double)rmax_col);
                 hb = (float)(A.b1 * R * R + A.b2 * R + A.b3);  // half the band
             }
-            const float nb = sb_na[buf][col];
+            // the accumulator starts at -|b|^2/2 - (|a|^2 - e2)/2 and ends at
+            // dv = (e2 - est)/2 (-inf for rows and columns out of range)
+            const float nbh = -0.5f * sb_na[buf][col];
             f32x16_t acc;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) acc[r] = -0.5f * nb;
+            for (int r = 0; r < 16; ++r) acc[r] = nbh - mid_r[r];
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
                 const bf16x8_t bhi = sb_hi[buf][col * ROWU + ks * 2 + half];
@@ -784,13 +820,18 @@ __global__ __launch_bounds__(256, KP >= 128 ? 2 : 3) void k_mf_count(MfmaArgs A,
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[ks], blo, acc, 0, 0, 0);
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[ks], bhi, acc, 0, 0, 0);
             }
-            // dv = acc - (|a|^2 - e2)/2 = (e2 - est)/2: dv >= hb is a neighbour,
-            // dv < -hb is not, the rest (the guard band) the exact predicate decides
+            // dv >= hb is a neighbour, dv < -hb is not, the rest (the guard
+            // band) the exact predicate decides.  A block whose largest dv is
+            // below -hb (no neighbour, no band element) needs nothing more.
+            float amax = acc[0];
+#pragma unroll
+            for (int r = 1; r < 16; ++r) amax = fmaxf(amax, acc[r]);
             int32_t ccnt = 0;
+            if (__ballot(amax >= -hb) != 0) {
             float amin = INF;
 #pragma unroll
             for (int r = 0; r < 16; r += 2) {
-                const float dv0 = acc[r] - mid_r[r], dv1 = acc[r + 1] - mid_r[r + 1];
+                const float dv0 = acc[r], dv1 = acc[r + 1];
                 const int32_t in0 = dv0 >= hb ? 1 : 0, in1 = dv1 >= hb ? 1 : 0;
                 rcnt[r] += in0;
                 rcnt[r + 1] += in1;
@@ -802,7 +843,7 @@ __global__ __launch_bounds__(256, KP >= 128 ? 2 : 3) void k_mf_count(MfmaArgs A,
                 uint32_t bm = 0, hit = 0;
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    const float dv = acc[r] - mid_r[r];
+                    const float dv = acc[r];
                     bm |= (uint32_t)(dv < hb && dv >= -hb) << r;
                 }
                 const float *pc = A.xs + (int64_t)(bcb + col) * A.d;
@@ -817,19 +858,36 @@ __global__ __launch_bounds__(256, KP >= 128 ? 2 : 3) void k_mf_count(MfmaArgs A,
                 for (int r = 0; r < 16; ++r) rcnt[r] += (hit >> r) & 1;
             }
             // the pair holds a pair within eps: the label passes visit it
-            if (__ballot(ccnt != 0) != 0 && lane == 0) A.pflag[sb_ci[buf]] = 1;
+            if (__ballot(ccnt != 0) != 0 && lane == 0) sb_edge = 1;
             if (!diag) {
                 ccnt += __shfl_xor(ccnt, 32, 64);
-                if (half == 0 && ccnt != 0) atomicAdd(&sb_col[buf][col], ccnt);
+                if (half == 0 && ccnt != 0) atomicAdd(&sb_cacc[(bcb - bcs) % kColWin + col], ccnt);
+            }
             }
         }
         if (more) publish(buf ^ 1);
         __syncthreads();
-        // column counts of the block just computed (its stage is rewritten
-        // only after the next barrier)
-        if (!diag && threadIdx.x < 32) {
-            const int32_t v = sb_col[buf][threadIdx.x];
-            if (v != 0) atomicAdd(A.nn_s + bcb + threadIdx.x, v);
+        // leaving the window (another cell, the next kColWin columns, or the
+        // end): its column counts and pair flag go to global memory
+        const int ncell = __builtin_amdgcn_readfirstlane(sb_cell[buf ^ 1]);
+        const int ncb = __builtin_amdgcn_readfirstlane(sb_cb[buf ^ 1]);
+        const int ncs = __builtin_amdgcn_readfirstlane(sb_cs[buf ^ 1]);
+        if (!more || ncell != o || (ncb - ncs) / kColWin != (bcb - bcs) / kColWin) {
+            if (!diag) {
+                const int32_t wb = bcs + (bcb - bcs) / kColWin * kColWin;
+                for (int t = threadIdx.x; t < kColWin; t += blockDim.x) {
+                    const int32_t v = sb_cacc[t];
+                    if (v != 0) {
+                        atomicAdd(A.nn_s + wb + t, v);
+                        sb_cacc[t] = 0;
+                    }
+                }
+            }
+            if (threadIdx.x == 0 && sb_edge) {
+                A.pflag[bci] = 1;
+                sb_edge = 0;
+            }
+            __syncthreads();
         }
         if (!more) break;
         buf ^= 1;
@@ -1004,11 +1062,14 @@ static MfmaArgs mfma_args(Ctx &c) {
     const int kp = kp_for(c.d);
     const double u = std::ldexp(1.0, -24);
     // the last terms cover the count pass: its accumulator starts at
-    // -|b|^2/2 (one more rounding of a value <= R^2/2 per MFMA, 3 kp / 16
-    // MFMAs) and it compares acc - (|a|^2 - e2)/2 (two more roundings)
-    a.b1 = 2.0 * (3.1 * std::ldexp(1.0, -16) + 3.0 * kp * u) + (2.0 * kp + 16.0) * u + (3.0 * kp / 16.0 + 4.0) * u;
+    // -|b|^2/2 - (|a|^2 - e2)/2 (the threshold folded in) and every value it
+    // takes is at most |a||b| + (|a|^2 + |b|^2)/2 + e2/2 <= 2 R^2 + e2/2 in
+    // magnitude: one rounding of that per MFMA (3 kp / 16 of them) and four
+    // more for the start value and (|a|^2 - e2)/2
+    const double cnt_r = (3.0 * kp / 16.0 + 4.0) * u;
+    a.b1 = 2.0 * (3.1 * std::ldexp(1.0, -16) + 3.0 * kp * u) + (2.0 * kp + 16.0) * u + 2.0 * cnt_r;
     a.b2 = std::ldexp(1.0, -21) * std::sqrt((double)a.e2 * 1.01);
-    a.b3 = (c.d + 3) * u * (double)a.e2;
+    a.b3 = (c.d + 3) * u * (double)a.e2 + 0.5 * cnt_r * (double)a.e2;
     return a;
 }
 
