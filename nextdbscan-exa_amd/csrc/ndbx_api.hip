@@ -58,6 +58,8 @@ static void open_common(Ctx &c, const float *coords, int64_t n_total, uint32_t f
     NDBX_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     c.prof.on = (flags & NDBX_PROFILE) != 0;
     if (const char *gc = std::getenv("NDBX_GROUP_CELLS")) c.group_cells = std::max(1, std::atoi(gc));
+    // fixed guard-band queue capacity of the MFMA count pass (tests of its overflow path)
+    if (const char *bq = std::getenv("NDBX_BAND_QUEUE")) c.bq_fixed = std::max(1ll, std::atoll(bq));
     c.n = n_total;
     c.flags = flags;
     const size_t bytes = (size_t)n_total * (size_t)c.d * sizeof(float);

@@ -269,6 +269,26 @@ __device__ __forceinline__ bool exact_pred(const float *__restrict__ p, const fl
     return len <= e2;
 }
 
+// The same predicate, coordinates read 16 bytes at a time (d % 4 == 0, rows
+// 16-byte aligned); the sum is still taken in ascending k.
+__device__ __forceinline__ bool exact_pred4(const float *__restrict__ p, const float *__restrict__ q, int d,
+                                            float e2) {
+    const float4 *p4 = reinterpret_cast<const float4 *>(p), *q4 = reinterpret_cast<const float4 *>(q);
+    float len = 0.0f;
+    for (int k = 0; k < d / 4; ++k) {
+        const float4 a = p4[k], b = q4[k];
+        float df = a.x - b.x;
+        len = __fadd_rn(len, __fmul_rn(df, df));
+        df = a.y - b.y;
+        len = __fadd_rn(len, __fmul_rn(df, df));
+        df = a.z - b.z;
+        len = __fadd_rn(len, __fmul_rn(df, df));
+        df = a.w - b.w;
+        len = __fadd_rn(len, __fmul_rn(df, df));
+    }
+    return len <= e2;
+}
+
 struct MfmaArgs {
     const int4 *tiles;
     int64_t n_tiles;
@@ -291,6 +311,9 @@ struct MfmaArgs {
     unsigned long long *audit;        // [band pairs, fp64 flips, filter flips], pass 3
     float *audit_err;                 // max |est - fp64 len|, pass 3
     int audit_stride;
+    int4 *bq;          // count pass: guard-band pairs (row, column, candidate entry, own pair)
+    unsigned long long *bq_count;
+    int64_t bq_cap;
 };
 
 // Work groups: up to 4 consecutive row tiles of one cell, one workgroup (4
@@ -666,6 +689,7 @@ __device__ __forceinline__ int64_t mf_group_of(int64_t b, int chunk) {
 // there would hold up the next block's loads, which wait on the same
 // counter).
 constexpr int kColWin = 2048;
+constexpr int kBandQ = 64;  // guard-band pairs queued per wave in LDS
 
 template <int KP>
 __global__ __launch_bounds__(256, KP >= 128 ? 2 : 3) void k_mf_count(MfmaArgs A, int chunk) {
@@ -678,6 +702,7 @@ __global__ __launch_bounds__(256, KP >= 128 ? 2 : 3) void k_mf_count(MfmaArgs A,
     __shared__ int32_t sb_cacc[kColWin];  // column counts of the current window
     __shared__ int32_t sb_cell[2], sb_cb[2], sb_ci[2], sb_cs[2], sb_edge;
     __shared__ float sb_rmax[2];
+    __shared__ int4 sb_q[4][kBandQ];  // per wave: guard-band pairs not yet in the global queue
     const int64_t gidx = mf_group_of(blockIdx.x, chunk);
     if (gidx >= A.n_groups) return;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -713,6 +738,26 @@ __global__ __launch_bounds__(256, KP >= 128 ? 2 : 3) void k_mf_count(MfmaArgs A,
     for (int o = 32; o >= 1; o >>= 1) rmax_row = fmaxf(rmax_row, __shfl_xor(rmax_row, o, 64));
     rmax_row = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(rmax_row)));
 
+    // This wave's queued guard-band pairs to the global queue (one atomic);
+    // past its capacity they are decided here.
+    int qn = 0;
+    auto band_flush = [&]() {
+        if (qn == 0) return;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(A.bq_count, (unsigned long long)qn);
+        base = __shfl(base, 0, 64);
+        for (int i = lane; i < qn; i += 64) {
+            const int4 e = sb_q[wave][i];
+            if (base + (unsigned long long)i < (unsigned long long)A.bq_cap) {
+                A.bq[base + i] = e;
+            } else if (exact_pred(A.xs + (int64_t)e.x * A.d, A.xs + (int64_t)e.y * A.d, A.d, A.e2)) {
+                atomicAdd(A.nn_s + e.x, 1);
+                if (!e.w) atomicAdd(A.nn_s + e.y, 1);
+                A.pflag[e.z] = 1;
+            }
+        }
+        qn = 0;
+    };
     // block sequence: the 32-column blocks of the cells this cell takes
     const int c_beg = ld_const(A.cand_off + cell), c_end = ld_const(A.cand_off + cell + 1);
     int ci = c_beg, cb = 0, cs = 0, ce = 0, co = -1;
@@ -839,23 +884,45 @@ __global__ __launch_bounds__(256, KP >= 128 ? 2 : 3) void k_mf_count(MfmaArgs A,
                 amin = fminf(amin, fminf(fabsf(dv0), fabsf(dv1)));
             }
             if (__ballot(amin <= hb) != 0) {
-                // band elements (-hb <= dv < hb) one at a time per lane
-                uint32_t bm = 0, hit = 0;
+                // guard-band elements (-hb <= dv < hb) go to this wave's queue
+                uint32_t bm = 0;
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
                     const float dv = acc[r];
                     bm |= (uint32_t)(dv < hb && dv >= -hb) << r;
                 }
-                const float *pc = A.xs + (int64_t)(bcb + col) * A.d;
-                while (bm) {
-                    const int r = __builtin_ctz(bm);
-                    bm &= bm - 1;
-                    const int tr = 8 * (r >> 2) + 4 * half + (r & 3);
-                    if (exact_pred(A.xs + (int64_t)(row0 + tr) * A.d, pc, A.d, A.e2)) hit |= 1u << r;
-                }
-                ccnt += __builtin_popcount(hit);
+                const int cnt = __builtin_popcount(bm);
+                int incl = cnt;
 #pragma unroll
-                for (int r = 0; r < 16; ++r) rcnt[r] += (hit >> r) & 1;
+                for (int o2 = 1; o2 < 64; o2 <<= 1) {
+                    const int t2 = __shfl_up(incl, o2, 64);
+                    if (lane >= o2) incl += t2;
+                }
+                const int total = __builtin_amdgcn_readlane(incl, 63);
+                if (qn + total > kBandQ) band_flush();
+                if (total > kBandQ) {
+                    // too many for the queue: decided here, one lane per column
+                    uint32_t hit = 0;
+                    const float *pc = A.xs + (int64_t)(bcb + col) * A.d;
+                    while (bm) {
+                        const int r = __builtin_ctz(bm);
+                        bm &= bm - 1;
+                        const int tr = 8 * (r >> 2) + 4 * half + (r & 3);
+                        if (exact_pred(A.xs + (int64_t)(row0 + tr) * A.d, pc, A.d, A.e2)) hit |= 1u << r;
+                    }
+                    ccnt += __builtin_popcount(hit);
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) rcnt[r] += (hit >> r) & 1;
+                } else {
+                    int pos = qn + incl - cnt;
+                    while (bm) {
+                        const int r = __builtin_ctz(bm);
+                        bm &= bm - 1;
+                        const int tr = 8 * (r >> 2) + 4 * half + (r & 3);
+                        sb_q[wave][pos++] = make_int4(row0 + tr, bcb + col, bci, diag ? 1 : 0);
+                    }
+                    qn += total;
+                }
             }
             // the pair holds a pair within eps: the label passes visit it
             if (__ballot(ccnt != 0) != 0 && lane == 0) sb_edge = 1;
@@ -893,6 +960,7 @@ __global__ __launch_bounds__(256, KP >= 128 ? 2 : 3) void k_mf_count(MfmaArgs A,
         buf ^= 1;
     }
     if (!wave_on) return;
+    band_flush();
     // each row over the 32 lanes (columns) of its half
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
@@ -905,6 +973,27 @@ __global__ __launch_bounds__(256, KP >= 128 ? 2 : 3) void k_mf_count(MfmaArgs A,
         for (int r = 0; r < 16; ++r) {
             const int tr = 8 * (r >> 2) + 4 * half + (r & 3);
             if (tr < nrows && rcnt[r] != 0) atomicAdd(A.nn_s + row0 + tr, rcnt[r]);
+        }
+    }
+}
+
+// Guard-band pairs of the count pass, decided by the reference's float32
+// arithmetic on the original coordinates (data_process.cpp:443-447): a
+// neighbour adds to both counts (the row's only, in a cell's own pair) and
+// flags its cell pair.
+__global__ __launch_bounds__(256) void k_mf_band(const int4 *__restrict__ bq,
+                                                 const unsigned long long *__restrict__ bq_count, int64_t cap,
+                                                 const float *__restrict__ xs, int d, float e2,
+                                                 int32_t *__restrict__ nn_s, uint8_t *__restrict__ pflag) {
+    const int64_t nq = *bq_count < (unsigned long long)cap ? (int64_t)*bq_count : cap;
+    const bool v4 = (d & 3) == 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += (int64_t)gridDim.x * blockDim.x) {
+        const int4 e = bq[i];
+        const float *p = xs + (int64_t)e.x * d, *q = xs + (int64_t)e.y * d;
+        if (v4 ? exact_pred4(p, q, d, e2) : exact_pred(p, q, d, e2)) {
+            atomicAdd(nn_s + e.x, 1);
+            if (!e.w) atomicAdd(nn_s + e.y, 1);
+            pflag[e.z] = 1;
         }
     }
 }
@@ -1227,7 +1316,24 @@ void mfma_classify(Ctx &c) {
     MfmaArgs a = mfma_args(c);
     a.nn_s = c.nn_s.as<int32_t>();
     fill_i32(a.nn_s, 0, n, s, c.prof);
+    // guard-band queue: sized by the last run's count (pairs past the
+    // capacity are decided in the count pass itself)
+    c.bq_last = (int64_t)c.bq_last64;  // copied back after the previous run's count pass
+    c.bq_cap = std::max<int64_t>(c.bq_cap, std::max<int64_t>(1 << 20, 8 * n));
+    if (c.bq_last > c.bq_cap) c.bq_cap = c.bq_last + c.bq_last / 4;
+    c.bq_cap = std::min<int64_t>(c.bq_cap, 1ll << 30);
+    if (c.bq_fixed > 0) c.bq_cap = c.bq_fixed;
+    c.bq.ensure((size_t)c.bq_cap * 16);
+    a.bq = c.bq.as<int4>();
+    a.bq_count = reinterpret_cast<unsigned long long *>(c.misc.as<int32_t>() + 660);
+    a.bq_cap = c.bq_cap;
+    fill_u64(reinterpret_cast<uint64_t *>(a.bq_count), 0, 1, s, c.prof);
     launch_count(c, a, kp);
+    {
+        Launch L(c.prof, s, "mf_band");
+        k_mf_band<<<2048, 256, 0, s>>>(a.bq, a.bq_count, a.bq_cap, a.xs, a.d, a.e2, a.nn_s, a.pflag);
+    }
+    NDBX_HIP(hipMemcpyAsync(&c.bq_last64, a.bq_count, 8, hipMemcpyDeviceToHost, s));
     c.core_s.ensure(n);
     c.label_s.ensure(n * 4);
     c.parent.ensure(n * 4);

@@ -61,3 +61,27 @@ def test_mfma_matches_all_pairs(gpu_lib):
     np.testing.assert_array_equal(a[0], b[0])
     np.testing.assert_array_equal(a[1], b[1])
     assert a[2] == b[2]
+
+
+@pytest.mark.parametrize("cap", ["16", "100000000"])
+def test_mfma_guard_band_queue(gpu_lib, monkeypatch, cap):
+    """A 4-D integer lattice with eps = 1: every axis pair sits exactly at
+    eps^2, inside the guard band, so the count pass queues them; a queue of 16
+    entries decides the rest in the count pass itself (and blocks with more
+    than a wave's queue), a large one defers them all."""
+    g = np.arange(12, dtype=np.float32)
+    x = np.stack(np.meshgrid(g, g, g, g, indexing="ij"), -1).reshape(-1, 4).copy()
+    x = x[np.random.default_rng(7).permutation(len(x))]
+    eps = np.float32(1.0)
+    monkeypatch.setenv("NDBX_BAND_QUEUE", cap)
+    print(f"case mfma lattice4d band queue {cap}", flush=True)
+    with gpu_lib.Context(x, eps, 6) as c:
+        c.classify().label()
+        nn, lab, meta, path = c.nn(), c.labels(), c.meta(), c.path()
+        c.classify().label()  # a second run reuses the queue
+        assert np.array_equal(c.nn(), nn)
+    nn_o, lab_o, meta_o = O.dbscan(x, eps, 6)
+    assert path == "mfma-filter"
+    np.testing.assert_array_equal(nn, nn_o)
+    np.testing.assert_array_equal(lab, lab_o)
+    assert meta == meta_o
