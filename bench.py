@@ -110,6 +110,7 @@ def main():
     ap.add_argument("--n-inst", type=int, default=4096, help="instances of the batched workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU form comparison")
+    ap.add_argument("--no-audit", action="store_true", help="skip the float64 audit of the MFMA filter")
     ap.add_argument("--graph", action="store_true",
                     help="batched workload: also time graph replays (NDBX_GRAPH) and report them as value")
     ap.add_argument("--autotune", action="store_true",
@@ -211,7 +212,7 @@ def main():
                 plain_nn = pc.nn()
             extra["plain_gpu"] = {"ms_per_step": plain_ms, "value": n / (plain_ms * 1e-3),
                                   "form": "all-pairs float32 kernels (FORCE_BRUTE), 1 timed step"}
-        if d > 3:
+        if d > 3 and not args.no_audit:
             with ndbx.Context(x, eps, m, flags=ndbx.FP64_AUDIT) as ac:
                 ac.classify()
                 extra["fp64_audit"] = ac.audit()

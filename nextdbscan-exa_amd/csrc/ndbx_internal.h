@@ -161,7 +161,7 @@ struct Ctx {
     int64_t graph_replays = 0, graph_reruns = 0;
 
     // brute / mfma path state
-    DevBuf b_parent, b_rootmin, pflag, bq;
+    DevBuf b_parent, b_rootmin, pflag, bq, cgroups;
     int64_t bq_cap = 0, bq_last = 0;  // guard-band queue: capacity, last run's count
     unsigned long long bq_last64 = 0;  // (copied back after the count pass)
     int64_t bq_fixed = 0;              // > 0: capacity fixed (NDBX_BAND_QUEUE)

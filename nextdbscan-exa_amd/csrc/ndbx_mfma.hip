@@ -316,21 +316,23 @@ struct MfmaArgs {
     int64_t bq_cap;
 };
 
-// Work groups: up to 4 consecutive row tiles of one cell, one workgroup (4
-// waves, one tile each).  The waves share every 32-column B block through LDS.
+// Work groups: up to `per` consecutive row tiles of one cell, one workgroup
+// (one wave per tile: 4 for the label passes, 8 for the count pass).  The
+// waves share every 32-column B block through LDS.
 __global__ __launch_bounds__(256) void k_groups(const int32_t *__restrict__ toff, int64_t nc,
-                                                const int32_t *__restrict__ goff, int4 *__restrict__ groups) {
+                                                const int32_t *__restrict__ goff, int per,
+                                                int4 *__restrict__ groups) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     const int32_t t0 = toff[c], t1 = toff[c + 1];
     int32_t g = goff[c];
-    for (int32_t t = t0; t < t1; t += 4) groups[g++] = make_int4((int32_t)c, t, t1 - t < 4 ? t1 - t : 4, 0);
+    for (int32_t t = t0; t < t1; t += per) groups[g++] = make_int4((int32_t)c, t, t1 - t < per ? t1 - t : per, 0);
 }
 
-__global__ __launch_bounds__(256) void k_group_count(const int32_t *__restrict__ toff, int64_t nc,
+__global__ __launch_bounds__(256) void k_group_count(const int32_t *__restrict__ toff, int64_t nc, int per,
                                                      int32_t *__restrict__ cnt) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < nc) cnt[c] = (toff[c + 1] - toff[c] + 3) / 4;
+    if (c < nc) cnt[c] = (toff[c + 1] - toff[c] + per - 1) / per;
 }
 
 // Each unordered pair of candidate cells is computed once: cell c takes
@@ -689,20 +691,26 @@ __device__ __forceinline__ int64_t mf_group_of(int64_t b, int chunk) {
 // there would hold up the next block's loads, which wait on the same
 // counter).
 constexpr int kColWin = 2048;
-constexpr int kBandQ = 64;  // guard-band pairs queued per wave in LDS
+constexpr int kBandQ = 64;       // guard-band pairs queued per wave in LDS
+// Row tiles (waves) per count workgroup.  Measured on the 1M 64-D workload:
+// 4 (3 waves per SIMD) 220 ms with 1.0 TB of L2 misses; 8 (2 per SIMD) 284 ms
+// with 0.40 TB; 12 (3 per SIMD) 234 ms with 0.30 TB: the pass is bound by
+// issue and latency, not by its candidate-block traffic.
+constexpr int kCountWaves = 4;
 
-template <int KP>
-__global__ __launch_bounds__(256, KP >= 128 ? 2 : 3) void k_mf_count(MfmaArgs A, int chunk) {
+template <int KP, int WAVES>
+__global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? 3 : 2) void k_mf_count(MfmaArgs A, int chunk) {
     constexpr int KS = KP / 16;
-    constexpr int UNITS = 32 * (KP / 8);
+    constexpr int UNITS = 32 * (KP / 8);  // bf16x8 units of a block, per hi / lo
     constexpr int ROWU = KP / 8 + 1;
-    constexpr int UPT = (UNITS + 255) / 256;
+    constexpr int NT = 64 * WAVES;
+    constexpr int UPT = (2 * UNITS + NT - 1) / NT;  // hi and lo units per thread
     __shared__ bf16x8_t sb_hi[2][32 * ROWU], sb_lo[2][32 * ROWU];
     __shared__ float sb_na[2][32];
     __shared__ int32_t sb_cacc[kColWin];  // column counts of the current window
     __shared__ int32_t sb_cell[2], sb_cb[2], sb_ci[2], sb_cs[2], sb_edge;
     __shared__ float sb_rmax[2];
-    __shared__ int4 sb_q[4][kBandQ];  // per wave: guard-band pairs not yet in the global queue
+    __shared__ int4 sb_q[WAVES][kBandQ];  // per wave: guard-band pairs not yet in the global queue
     const int64_t gidx = mf_group_of(blockIdx.x, chunk);
     if (gidx >= A.n_groups) return;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -782,21 +790,19 @@ __global__ __launch_bounds__(256, KP >= 128 ? 2 : 3) void k_mf_count(MfmaArgs A,
     // accumulator is -inf whatever the products are.
     // (The norm is selected only when published: a select right after its
     // load would wait for every load of the fetch.)
-    bf16x8_t pf_hi[UPT], pf_lo[UPT];
+    bf16x8_t pf[UPT];
     float pf_na = INF, pf_rmax = 0.0f;
     bool pf_ok = false;
     const int64_t n_last = A.n - 1;
     auto fetch = [&]() {
 #pragma unroll
         for (int q = 0; q < UPT; ++q) {
-            const int u = threadIdx.x + q * 256;
-            const int bc = u / (KP / 8), kk = u - bc * (KP / 8);
+            const int u = threadIdx.x + q * NT;
+            const int uu = u < UNITS ? u : u - UNITS;  // units [0, UNITS): hi, then lo
+            const int bc = uu / (KP / 8), kk = uu - bc * (KP / 8);
             const int64_t pt = cb + bc < n_last ? cb + bc : n_last;
             const int64_t src = pt * (KP / 8) + kk;
-            if (u < UNITS) {
-                pf_hi[q] = hi8[src];
-                pf_lo[q] = lo8[src];
-            }
+            if (u < 2 * UNITS) pf[q] = u < UNITS ? hi8[src] : lo8[src];
         }
         const int jn = cb + (int)(threadIdx.x & 31);
         pf_na = A.na[jn < n_last ? jn : n_last];
@@ -806,12 +812,11 @@ __global__ __launch_bounds__(256, KP >= 128 ? 2 : 3) void k_mf_count(MfmaArgs A,
     auto publish = [&](int buf) {
 #pragma unroll
         for (int q = 0; q < UPT; ++q) {
-            const int u = threadIdx.x + q * 256;
-            if (u < UNITS) {
-                const int bc = u / (KP / 8), kk = u - bc * (KP / 8);
-                sb_hi[buf][bc * ROWU + kk] = pf_hi[q];
-                sb_lo[buf][bc * ROWU + kk] = pf_lo[q];
-            }
+            const int u = threadIdx.x + q * NT;
+            const int uu = u < UNITS ? u : u - UNITS;
+            const int bc = uu / (KP / 8), kk = uu - bc * (KP / 8);
+            if (u < UNITS) sb_hi[buf][bc * ROWU + kk] = pf[q];
+            else if (u < 2 * UNITS) sb_lo[buf][bc * ROWU + kk] = pf[q];
         }
         if (threadIdx.x < 32) sb_na[buf][threadIdx.x] = pf_ok ? pf_na : INF;
         if (threadIdx.x == 0) {
@@ -1113,18 +1118,20 @@ static void launch_pass(Ctx &c, const MfmaArgs &a, int kp, const char *name) {
     }
 }
 
-// Symmetric count pass: groups in XCD chunks of kCountChunk.
+// Symmetric count pass: groups of kCountWaves row tiles, in XCD chunks of
+// kCountChunk groups.
 constexpr int kCountChunk = 16;
 static void launch_count(Ctx &c, const MfmaArgs &a, int kp) {
     if (a.n_groups == 0) return;
     const int64_t per = 8 * kCountChunk;
     const unsigned g = (unsigned)(ceil_div(a.n_groups, per) * per);
+    constexpr int NT = 64 * kCountWaves;
     Launch L(c.prof, c.stream, "mf_count");
     switch (kp) {
-        case 16: k_mf_count<16><<<g, 256, 0, c.stream>>>(a, kCountChunk); break;
-        case 32: k_mf_count<32><<<g, 256, 0, c.stream>>>(a, kCountChunk); break;
-        case 64: k_mf_count<64><<<g, 256, 0, c.stream>>>(a, kCountChunk); break;
-        default: k_mf_count<128><<<g, 256, 0, c.stream>>>(a, kCountChunk); break;
+        case 16: k_mf_count<16, kCountWaves><<<g, NT, 0, c.stream>>>(a, kCountChunk); break;
+        case 32: k_mf_count<32, kCountWaves><<<g, NT, 0, c.stream>>>(a, kCountChunk); break;
+        case 64: k_mf_count<64, kCountWaves><<<g, NT, 0, c.stream>>>(a, kCountChunk); break;
+        default: k_mf_count<128, kCountWaves><<<g, NT, 0, c.stream>>>(a, kCountChunk); break;
     }
 }
 
@@ -1298,7 +1305,7 @@ void mfma_classify(Ctx &c) {
     // work groups of up to 4 row tiles of one cell
     {
         Launch L(c.prof, s, "mf_row_tiles");
-        k_group_count<<<ncb, 256, 0, s>>>(c.ctile_off.as<int32_t>(), nc, c.nranges.as<int32_t>());
+        k_group_count<<<ncb, 256, 0, s>>>(c.ctile_off.as<int32_t>(), nc, 4, c.nranges.as<int32_t>());
     }
     c.cid.ensure((nc + 1) * 4);
     scan_exclusive_i32(c.nranges.as<int32_t>(), c.cid.as<int32_t>(), nc, d_total, c.scratch, s, c.prof);
@@ -1308,7 +1315,28 @@ void mfma_classify(Ctx &c) {
     c.tiles.ensure(c.n_label_tiles * 16 + 16);
     {
         Launch L(c.prof, s, "mf_row_tiles");
-        k_groups<<<ncb, 256, 0, s>>>(c.ctile_off.as<int32_t>(), nc, c.cid.as<int32_t>(), c.tiles.as<int4>());
+        k_groups<<<ncb, 256, 0, s>>>(c.ctile_off.as<int32_t>(), nc, c.cid.as<int32_t>(), 4, c.tiles.as<int4>());
+    }
+    // count-pass groups: up to kCountWaves row tiles of one cell (the label
+    // passes' groups when that is 4)
+    int64_t n_count_groups = c.n_label_tiles;
+    const int4 *count_groups = c.tiles.as<int4>();
+    if (kCountWaves != 4) {
+        {
+            Launch L(c.prof, s, "mf_row_tiles");
+            k_group_count<<<ncb, 256, 0, s>>>(c.ctile_off.as<int32_t>(), nc, kCountWaves, c.nranges.as<int32_t>());
+        }
+        scan_exclusive_i32(c.nranges.as<int32_t>(), c.cid.as<int32_t>(), nc, d_total, c.scratch, s, c.prof);
+        NDBX_HIP(hipMemcpyAsync(&h_total, d_total, 4, hipMemcpyDeviceToHost, s));
+        c.sync();
+        n_count_groups = h_total;
+        c.cgroups.ensure(n_count_groups * 16 + 16);
+        {
+            Launch L(c.prof, s, "mf_row_tiles");
+            k_groups<<<ncb, 256, 0, s>>>(c.ctile_off.as<int32_t>(), nc, c.cid.as<int32_t>(), kCountWaves,
+                                         c.cgroups.as<int4>());
+        }
+        count_groups = c.cgroups.as<int4>();
     }
     c.pair_tests = (int64_t)h_pts;  // point pairs of the candidate cell pairs (read after the syncs above)
     // count
@@ -1328,7 +1356,12 @@ void mfma_classify(Ctx &c) {
     a.bq_count = reinterpret_cast<unsigned long long *>(c.misc.as<int32_t>() + 660);
     a.bq_cap = c.bq_cap;
     fill_u64(reinterpret_cast<uint64_t *>(a.bq_count), 0, 1, s, c.prof);
-    launch_count(c, a, kp);
+    {
+        MfmaArgs ac = a;
+        ac.groups = count_groups;
+        ac.n_groups = n_count_groups;
+        launch_count(c, ac, kp);
+    }
     {
         Launch L(c.prof, s, "mf_band");
         k_mf_band<<<2048, 256, 0, s>>>(a.bq, a.bq_count, a.bq_cap, a.xs, a.d, a.e2, a.nn_s, a.pflag);
@@ -1409,7 +1442,7 @@ void mfma_label(Ctx &c) {
     scan_exclusive_i32(bcnt, btoff, nc, btoff + nc, c.scratch, s, c.prof);
     {
         Launch L(c.prof, s, "mf_border_rows");
-        k_group_count<<<ncb, 256, 0, s>>>(btoff, nc, bcnt);
+        k_group_count<<<ncb, 256, 0, s>>>(btoff, nc, 4, bcnt);
     }
     scan_exclusive_i32(bcnt, bgoff, nc, bgoff + nc, c.scratch, s, c.prof);
     int32_t h_t[2] = {0, 0};
@@ -1421,7 +1454,7 @@ void mfma_label(Ctx &c) {
     {
         Launch L(c.prof, s, "mf_border_rows");
         k_btiles<<<ncb, 256, 0, s>>>(bpos, c.cell_start.as<int32_t>(), btoff, nc, btiles);
-        k_groups<<<ncb, 256, 0, s>>>(btoff, nc, bgoff, bgroups);
+        k_groups<<<ncb, 256, 0, s>>>(btoff, nc, bgoff, 4, bgroups);
     }
     a.label_s = c.label_s.as<int32_t>();
     a.best_s = c.cell_label.as<int32_t>();
