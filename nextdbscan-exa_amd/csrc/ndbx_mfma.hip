@@ -311,6 +311,8 @@ struct MfmaArgs {
     unsigned long long *audit;        // [band pairs, fp64 flips, filter flips], pass 3
     float *audit_err;                 // max |est - fp64 len|, pass 3
     int audit_stride;
+    int union_phase;          // pass 1: 0 the cells' own pairs, 1 the other flagged pairs
+    const int32_t *cell_rep;  // pass 1, phase 1: a core of the cell when all its cores share a root, else -1
     int4 *bq;          // count pass: guard-band pairs (row, column, candidate entry, own pair)
     unsigned long long *bq_count;
     int64_t bq_cap;
@@ -426,9 +428,23 @@ __global__ __launch_bounds__(256) void k_mfma_pass(MfmaArgs A) {
     constexpr int UPT = (UNITS + 255) / 256;  // units per thread
     const int c_beg = A.cand_off[cell], c_end = A.cand_off[cell + 1];
     int ci = c_beg, cb = 0, ce = 0, cs = 0;
+    // Union, phase 1: a pair whose cells each have all their cores in one
+    // set, the same set, holds no union to make (read-only root walks).
+    auto root_of = [&](int32_t x) -> int32_t {
+        for (int32_t p = mf_ld(A.parent + x); p != x; p = mf_ld(A.parent + x)) x = p;
+        return x;
+    };
+    const int32_t rep_c = (PASS == 1 && A.union_phase == 1) ? A.cell_rep[cell] : -1;
+    auto same_set = [&](int o) -> bool {
+        if (PASS != 1 || A.union_phase != 1 || rep_c < 0) return false;
+        const int32_t rep_o = A.cell_rep[o];
+        return rep_o >= 0 && root_of(rep_c) == root_of(rep_o);
+    };
     // the candidate cells a pass visits (an empty range for the others)
     auto wanted = [&](int ci_, int o) -> bool {
         if (PASS == 0 || PASS == 3) return true;
+        if (PASS == 1 && (o == cell) != (A.union_phase == 0)) return false;
+        if (PASS == 1 && same_set(o)) return false;
         if (mf_takes(cell, o)) return A.pflag[ci_] != 0;
         if (PASS == 1) return false;
         int lo_ = A.cand_off[o], hi_ = A.cand_off[o + 1];  // entry of `cell` in o's list
@@ -453,6 +469,8 @@ __global__ __launch_bounds__(256) void k_mfma_pass(MfmaArgs A) {
         while (ci_ < c_end) {
             if (PASS == 1 && cb_ + 31 < min_row_point && A.cand[ci_] == cell)
                 cb_ += 32 * ((min_row_point - cb_) / 32);
+            // a pair whose cells have come to share one set since its first block
+            if (PASS == 1 && cb_ > cs_ && cb_ < ce_ && same_set(A.cand[ci_])) cb_ = ce_;
             if (cb_ < ce_) return;
             ++ci_;
             if (ci_ < c_end) load_cell(ci_, cb_, cs_, ce_);
@@ -1005,10 +1023,13 @@ __global__ __launch_bounds__(256) void k_mf_band(const int4 *__restrict__ bq,
 
 // Non-core rows for the border pass: compacted list (sorted order, so grouped
 // by cell), per-cell tile counts.
-__global__ __launch_bounds__(256) void k_noncore_flag(const uint8_t *__restrict__ core_s, int64_t n,
+// A non-core point whose only neighbour is itself (nn = 1) is within eps of
+// no core: it keeps NO_CLUSTER without a border row.
+__global__ __launch_bounds__(256) void k_noncore_flag(const uint8_t *__restrict__ core_s,
+                                                      const int32_t *__restrict__ nn_s, int64_t n,
                                                       int32_t *__restrict__ f) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) f[i] = !core_s[i];
+    if (i < n) f[i] = !core_s[i] && nn_s[i] > 1;
 }
 __global__ __launch_bounds__(256) void k_noncore_list(const int32_t *__restrict__ f, const int32_t *__restrict__ pos,
                                                       int64_t n, int32_t *__restrict__ list) {
@@ -1034,6 +1055,39 @@ __global__ __launch_bounds__(256) void k_btiles(const int32_t *__restrict__ pos,
 }
 
 // ------------------------------------------------------------- labels --
+// Per cell: a core of the cell when all its cores have one root (after the
+// union of the cells' own pairs), else -1 (no core, or several sets).
+__global__ __launch_bounds__(256) void k_mf_cell_rep(const uint8_t *__restrict__ core_s,
+                                                     const int32_t *__restrict__ cell_start, int64_t nc,
+                                                     const int32_t *__restrict__ parent, int32_t *__restrict__ rep) {
+    const int64_t c = blockIdx.x;
+    if (c >= nc) return;
+    __shared__ int32_t first, root0, mixed;
+    const int32_t b = cell_start[c], e = cell_start[c + 1];
+    if (threadIdx.x == 0) {
+        first = INT_MAX;
+        mixed = 0;
+    }
+    __syncthreads();
+    for (int32_t i = b + threadIdx.x; i < e; i += blockDim.x)
+        if (core_s[i]) atomicMin(&first, i);
+    __syncthreads();
+    if (first == INT_MAX) {
+        if (threadIdx.x == 0) rep[c] = -1;
+        return;
+    }
+    auto root = [&](int32_t x) {
+        while (parent[x] != x) x = parent[x];
+        return x;
+    };
+    if (threadIdx.x == 0) root0 = root(first);
+    __syncthreads();
+    for (int32_t i = b + threadIdx.x; i < e; i += blockDim.x)
+        if (core_s[i] && root(i) != root0) mixed = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) rep[c] = mixed ? -1 : first;
+}
+
 __global__ __launch_bounds__(256) void k_mf_core(const int32_t *__restrict__ nn_s, const uint32_t *__restrict__ perm,
                                                  int64_t n, int32_t m, int32_t *__restrict__ nn,
                                                  uint8_t *__restrict__ core, uint8_t *__restrict__ core_s,
@@ -1050,12 +1104,27 @@ __global__ __launch_bounds__(256) void k_mf_core(const int32_t *__restrict__ nn_
     parent[i] = (int32_t)i;
 }
 
+// Lowest input index of each set's cores, at its root.  Consecutive points
+// mostly share a root: when every core lane of a wave has the same one, the
+// wave takes the minimum first and makes one atomic (a few sets take all the
+// cores, so per-lane atomics would queue on a few addresses).
 __global__ __launch_bounds__(256) void k_mf_rootmin(const uint8_t *__restrict__ core_s,
                                                     const uint32_t *__restrict__ perm, int64_t n,
                                                     int32_t *__restrict__ parent, int32_t *__restrict__ root_min) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !core_s[i]) return;
-    atomicMin(root_min + mf_find(parent, (int32_t)i), (int32_t)perm[i]);
+    const bool core = i < n && core_s[i];
+    const int32_t r = core ? mf_find(parent, (int32_t)i) : -1;
+    const int32_t v = core ? (int32_t)perm[i] : INT_MAX;
+    const uint64_t cores = __ballot(core);
+    if (cores == 0) return;
+    const int32_t r0 = __shfl(r, __builtin_ctzll(cores), 64);
+    if (__ballot(core && r != r0) == 0) {
+        int32_t m = v;
+        for (int o = 32; o >= 1; o >>= 1) m = min(m, __shfl_xor(m, o, 64));
+        if ((threadIdx.x & 63) == 0) atomicMin(root_min + r0, m);
+    } else if (core) {
+        atomicMin(root_min + r, v);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_mf_label(const uint8_t *__restrict__ core_s, int64_t n,
@@ -1408,6 +1477,19 @@ void mfma_label(Ctx &c) {
     MfmaArgs a = mfma_args(c);
     a.core_s = c.core_s.as<uint8_t>();
     a.parent = c.parent.as<int32_t>();
+    // union: the cells' own pairs, then the other flagged pairs, skipped
+    // while both cells' cores already share one set
+    a.union_phase = 0;
+    launch_pass<1>(c, a, kp, "mf_union_own");
+    const int64_t nc = c.n_cells;
+    c.b_parent.ensure((size_t)nc * 4 + 4);
+    {
+        Launch L(c.prof, s, "mf_cell_rep");
+        k_mf_cell_rep<<<(unsigned)nc, 256, 0, s>>>(c.core_s.as<uint8_t>(), c.cell_start.as<int32_t>(), nc,
+                                                   c.parent.as<int32_t>(), c.b_parent.as<int32_t>());
+    }
+    a.union_phase = 1;
+    a.cell_rep = c.b_parent.as<int32_t>();
     launch_pass<1>(c, a, kp, "mf_union");
     c.comp_min.ensure(n * 4);
     fill_i32(c.comp_min.as<int32_t>(), INT_MAX, n, s, c.prof);
@@ -1421,7 +1503,6 @@ void mfma_label(Ctx &c) {
     c.cell_label.ensure(n * 4);
     fill_i32(c.cell_label.as<int32_t>(), NDBX_NO_CLUSTER, n, s, c.prof);
     // border rows: the non-core points, compacted (still grouped by cell)
-    const int64_t nc = c.n_cells;
     const int ncb = (int)ceil_div(nc, 256);
     c.flag.ensure((n + 1) * 4);
     c.cid.ensure((n + 1) * 4);
@@ -1429,7 +1510,7 @@ void mfma_label(Ctx &c) {
     int32_t *bpos = c.cid.as<int32_t>();
     {
         Launch L(c.prof, s, "mf_border_rows");
-        k_noncore_flag<<<nb, 256, 0, s>>>(c.core_s.as<uint8_t>(), n, c.flag.as<int32_t>());
+        k_noncore_flag<<<nb, 256, 0, s>>>(c.core_s.as<uint8_t>(), c.nn_s.as<int32_t>(), n, c.flag.as<int32_t>());
     }
     scan_exclusive_i32(c.flag.as<int32_t>(), bpos, n, bpos + n, c.scratch, s, c.prof);
     int32_t *btoff = c.ctile_off.as<int32_t>(), *bgoff = c.nranges.as<int32_t>();
