@@ -439,6 +439,7 @@ __global__ __launch_bounds__(256) void k_border(const int4 *__restrict__ ctiles,
                                                 const int32_t *__restrict__ cell_of,
                                                 const float *__restrict__ soa, const float *__restrict__ soa_core,
                                                 int64_t npad, const uint8_t *__restrict__ core_s,
+                                                const int32_t *__restrict__ nn_s,
                                                 const int32_t *__restrict__ cell_label,
                                                 const uint32_t *__restrict__ perm, int32_t *__restrict__ labels) {
     const int64_t nc = *ncp;
@@ -452,7 +453,8 @@ __global__ __launch_bounds__(256) void k_border(const int4 *__restrict__ ctiles,
     const int q = tile.z + (valid ? lane : 0);
     const bool is_core = valid && core_s[q];
     int32_t best = cell_label[cell_of[q]];  // NO_CLUSTER when the cell has no core
-    const bool need = valid && !is_core;
+    // a non-core point with no neighbour but itself (nn = 1) is within eps of no core
+    const bool need = valid && !is_core && nn_s[q] > 1;
     if (__ballot(need) != 0) {
         const InstGrid &g = ig[cell_key[tile.x] >> lin_bits];
         const float e2 = g.e2;
@@ -655,8 +657,8 @@ static void grid_label_d(Ctx &c) {
         k_border<D><<<(unsigned)ceil_div(nct, 4), 256, 0, s>>>(
             c.ctiles.as<int4>(), ntp, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig, c.lin_bits,
             c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), ncp, c.cell_of.as<int32_t>(), c.soa.as<float>(),
-            c.soa_core.as<float>(), npad, c.core_s.as<uint8_t>(), c.cell_label.as<int32_t>(), c.vals.as<uint32_t>(),
-            c.labels.as<int32_t>());
+            c.soa_core.as<float>(), npad, c.core_s.as<uint8_t>(), c.nn_s.as<int32_t>(), c.cell_label.as<int32_t>(),
+            c.vals.as<uint32_t>(), c.labels.as<int32_t>());
     }
     fill_i32(c.meta.as<int32_t>(), 0, 3 * c.n_inst, s, c.prof);
     {

@@ -125,3 +125,22 @@ def test_state_errors(gpu_lib):
     c.classify().label()
     assert c.meta() == (10, 0, 1)
     c.close()
+
+
+@pytest.mark.parametrize("group_cells", ["1", "3", "16"])
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_tile_shapes_against_oracle(gpu_lib, monkeypatch, d, group_cells):
+    """The symmetric count over tiles of one cell, of 3-cell runs (candidates
+    of the tile's own later cells tested per lane) and of 16-cell runs, with
+    dense, medium and sparse regions, so tiles of <= 16 and > 16 queries both
+    occur."""
+    x = synth.blobs(6000, d, 12, 0.0, 40.0, None, 0.1, 700 + d, sigma_range=(0.3, 3.0))
+    eps = synth.eps_f32({1: "0.1", 2: "0.6", 3: "1.2"}[d])
+    monkeypatch.setenv("NDBX_GROUP_CELLS", group_cells)
+    print(f"case tile shapes d={d} group_cells={group_cells}", flush=True)
+    _, nn, lab, meta, _, path = run(gpu_lib, x, eps, 10)
+    nn_o, lab_o, meta_o = O.dbscan(x, eps, 10)
+    assert path == "grid"
+    np.testing.assert_array_equal(nn, nn_o)
+    np.testing.assert_array_equal(lab, lab_o)
+    assert meta == meta_o
