@@ -20,6 +20,7 @@ namespace ndbx {
 constexpr int64_t kBoundsChunk = 1 << 16;
 constexpr int kFineBits = 6;  // 2-D: points of a cell in Morton order of a 64 x 64 sub-grid
 constexpr int kPartMin = 64;  // count kernel: cells with fewer points are taken whole, not by parts
+constexpr int kRunGap = 8;    // symmetric count: candidate runs closer than this are walked as one
 
 // ------------------------------------------------------------ bounds --
 __global__ __launch_bounds__(256) void k_bounds(const float *__restrict__ x, int d, int dg,
@@ -872,6 +873,16 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
 }
 
 // ------------------------------------------------------ symmetric count --
+// Quarter-cell parts: the top 2D bits of a point's sub-cell code, v in
+// [0, 4^D), in sort order.  Quarter coordinate of part v in dimension k:
+// d = 1, 3: bit k is the low bit and bit D + k the high bit of it (the 4^d
+// Morton code of k_keys); d = 2: the top 4 bits of the 64 x 64 code
+// interleave dimension 0 in the odd bits.
+template <int D>
+__device__ __forceinline__ int fine_q(int v, int k) {
+    if (D == 2) return k == 0 ? (((v >> 3) & 1) << 1) | ((v >> 1) & 1) : (((v >> 2) & 1) << 1) | (v & 1);
+    return (((v >> (D + k)) & 1) << 1) | ((v >> k) & 1);
+}
 // Grids without dense cells test every pair of distinct cells once.  Cells
 // are numbered in key order, (c2, c1, c0)-major, so a query's candidates of
 // higher cell index are exactly those of the stencil rows after the centre
@@ -972,7 +983,8 @@ __global__ __launch_bounds__(256) void k_count_sym(const int4 *__restrict__ ctil
                                                    const int32_t *__restrict__ cell_c, const int32_t *__restrict__ ncp,
                                                    const float *__restrict__ soa, int64_t npad,
                                                    int32_t *__restrict__ nn_s, int64_t *__restrict__ tests,
-                                                   const int32_t *__restrict__ part_off) {
+                                                   const int32_t *__restrict__ part_fine,
+                                                   const int32_t *__restrict__ big_id) {
     const int64_t nc = *ncp;
     const int64_t n_tiles = *ntp;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1016,7 +1028,7 @@ __global__ __launch_bounds__(256) void k_count_sym(const int4 *__restrict__ ctil
         cnt += count_run_sym<D, true>(qw, soa, npad, m_beg, m_end, e2, my_end, lane, nn_s);
         uint64_t rows = __ballot(pe > ps);
         const double r2o = search_r2(e2);
-        bool small_box = part_off != nullptr;
+        bool small_box = part_fine != nullptr;
 #pragma unroll
         for (int k = 0; k < D; ++k) small_box = small_box && (double)(hi[k] - lo[k]) * g.inv_s <= 1.0;
         while (rows) {
@@ -1024,18 +1036,18 @@ __global__ __launch_bounds__(256) void k_count_sym(const int4 *__restrict__ ctil
             rows &= rows - 1;
             if (small_box) {
                 const int a = __builtin_amdgcn_readlane(w.x, slot), e = __builtin_amdgcn_readlane(w.y, slot);
-                // the row's cells share dimensions 1..D-1: their half gaps once
-                const double s_ = 1.0 / g.inv_s, h = 0.5 * s_, slack = s_ * 9.5367431640625e-07;
-                double gy[2] = {0.0, 0.0}, gz[2] = {0.0, 0.0};
+                // Cells of kPartMin+ points are taken by their quarter-cell
+                // parts (4^D, Morton order) within reach of the tile box:
+                // lane v tests part v.  Dimensions 1..D-1 are the row's.
+                const double s_ = 1.0 / g.inv_s, qs = 0.25 * s_, slack = s_ * 9.5367431640625e-07;
+                double gyz = 0.0;
 #pragma unroll
                 for (int k = 1; k < D; ++k) {
-                    const double X = g.origin[k] + (double)cell_c[k * nc + a] * s_;
-#pragma unroll
-                    for (int b = 0; b < 2; ++b) {
-                        const double a0 = X + (double)b * h - slack, a1 = a0 + h + 2.0 * slack;
-                        const double gp = fmax(0.0, fmax(a0 - (double)hi[k], (double)lo[k] - a1));
-                        (k == 1 ? gy : gz)[b] = gp * gp;
-                    }
+                    const double a0 = g.origin[k] + (double)cell_c[k * nc + a] * s_ + (double)fine_q<D>(lane, k) * qs -
+                                      slack;
+                    const double a1 = a0 + qs + 2.0 * slack;
+                    const double gp = fmax(0.0, fmax(a0 - (double)hi[k], (double)lo[k] - a1));
+                    gyz += gp * gp;
                 }
                 int run = cell_start[a];  // pending run of whole cells
                 for (int B = a; B <= e; ++B) {
@@ -1045,33 +1057,37 @@ __global__ __launch_bounds__(256) void k_count_sym(const int4 *__restrict__ ctil
                     issued += b0 - run;
                     cnt += count_run_sym<D, false>(qw, soa, npad, run, b0, e2, my_end, lane, nn_s);
                     if (B == e) break;
-                    const double X = g.origin[0] + (double)cell_c[B] * s_;
-                    double gx[2];
-#pragma unroll
-                    for (int b = 0; b < 2; ++b) {
-                        const double a0 = X + (double)b * h - slack, a1 = a0 + h + 2.0 * slack;
-                        const double gp = fmax(0.0, fmax(a0 - (double)hi[0], (double)lo[0] - a1));
-                        gx[b] = gp * gp;
-                    }
-                    uint32_t mask = 0;
-#pragma unroll
-                    for (int o = 0; o < (1 << D); ++o) {
-                        double sum = gx[(o >> part_bit<D>(0)) & 1];
-                        if (D >= 2) sum += gy[(o >> part_bit<D>(1)) & 1];
-                        if (D >= 3) sum += gz[(o >> part_bit<D>(2)) & 1];
-                        if (sum <= r2o) mask |= 1u << o;
-                    }
-                    const int32_t *po = part_off + (int64_t)B * 8;
+                    const double a0 = g.origin[0] + (double)cell_c[B] * s_ + (double)fine_q<D>(lane, 0) * qs - slack;
+                    const double a1 = a0 + qs + 2.0 * slack;
+                    const double gp = fmax(0.0, fmax(a0 - (double)hi[0], (double)lo[0] - a1));
+                    uint64_t mask = __ballot(lane < (1 << (2 * D)) && gyz + gp * gp <= r2o);
+                    const int32_t *po = part_fine + (int64_t)big_id[B] * 64;
                     const int32_t b_end = cell_start[B + 1];
+                    // runs of parts in reach; a gap of at most kRunGap points
+                    // is tested rather than split the run (short runs cost
+                    // more per candidate than the few extra tests)
+                    int pj = -1, pe = -1;
                     while (mask) {
-                        const int o1 = __builtin_ctz(mask);
-                        const uint32_t above = ~((1u << o1) - 1u);
-                        const int o2 = (~mask & above) ? __builtin_ctz(~mask & above) : (1 << D);
-                        mask &= o2 < 32 ? ~((1u << o2) - 1u) : 0u;
+                        const int o1 = __builtin_ctzll(mask);
+                        const uint64_t gaps = ~mask & ~((1ull << o1) - 1ull);  // parts >= o1 not in reach
+                        const int o2 = gaps ? __builtin_ctzll(gaps) : 64;
+                        mask &= o2 < 64 ? ~((1ull << o2) - 1ull) : 0ull;
                         const int jj = po[o1];
-                        const int end = o2 < (1 << D) ? po[o2] : b_end;
-                        issued += end - jj;
-                        cnt += count_run_sym<D, false>(qw, soa, npad, jj, end, e2, my_end, lane, nn_s);
+                        const int end = o2 < (1 << (2 * D)) ? po[o2] : b_end;
+                        if (pj >= 0 && jj - pe <= kRunGap) {
+                            pe = end;
+                            continue;
+                        }
+                        if (pj >= 0) {
+                            issued += pe - pj;
+                            cnt += count_run_sym<D, false>(qw, soa, npad, pj, pe, e2, my_end, lane, nn_s);
+                        }
+                        pj = jj;
+                        pe = end;
+                    }
+                    if (pj >= 0) {
+                        issued += pe - pj;
+                        cnt += count_run_sym<D, false>(qw, soa, npad, pj, pe, e2, my_end, lane, nn_s);
                     }
                     run = b_end;
                 }
@@ -1213,6 +1229,42 @@ __global__ __launch_bounds__(256) void k_part_off(const uint64_t *__restrict__ k
     part_off[c * 8 + o] = lo;
 }
 
+// Cells of kPartMin+ points get a table of the first point of each of their
+// 4^D quarter-cell parts (part_fine[64 b + v], b the cell's index among such
+// cells, an exclusive scan of these flags).  At most n / kPartMin cells
+// qualify, so the table never exceeds 4 n bytes.
+__global__ __launch_bounds__(256) void k_big_flag(const int32_t *__restrict__ cell_start,
+                                                  const int32_t *__restrict__ ncp, int64_t ncap,
+                                                  int32_t *__restrict__ flag) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncap) return;
+    flag[c] = c < *ncp && cell_start[c + 1] - cell_start[c] >= kPartMin;
+}
+
+// One thread per point of a qualifying cell: the parts after the previous
+// point's, up to its own, start at it (the last point also closes the
+// parts after its own at the cell end).
+template <int D>
+__global__ __launch_bounds__(256) void k_part_fine(const uint64_t *__restrict__ keys, int sub_bits,
+                                                   const int32_t *__restrict__ cell_of,
+                                                   const int32_t *__restrict__ cell_start,
+                                                   const int32_t *__restrict__ big_id, int64_t n,
+                                                   int32_t *__restrict__ part_fine) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int32_t c = cell_of[p];
+    const int32_t b = cell_start[c], e = cell_start[c + 1];
+    if (e - b < kPartMin) return;
+    const int shift = sub_bits - 2 * D;
+    const uint64_t mask = (1ull << sub_bits) - 1;
+    const int vp = (int)((keys[p] & mask) >> shift);
+    const int vprev = p == b ? -1 : (int)((keys[p - 1] & mask) >> shift);
+    int32_t *t = part_fine + (int64_t)big_id[c] * 64;
+    for (int v = vprev + 1; v <= vp; ++v) t[v] = (int32_t)p;
+    if (p == e - 1)
+        for (int v = vp + 1; v < (1 << (2 * D)); ++v) t[v] = e;
+}
+
 // A captured run's count above its capacity: clamp it (so every later
 // launch stays in bounds) and flag the run for an eager rerun.
 __global__ void k_clamp_count(int32_t *__restrict__ count, int32_t cap, int32_t *__restrict__ status) {
@@ -1293,6 +1345,27 @@ static void grid_classify_d(Ctx &c) {
         k_part_off<D><<<(unsigned)ceil_div(nc << D, 256), 256, 0, s>>>(keys, c.sub_bits, c.cell_start.as<int32_t>(),
                                                                         ncp, c.part_off.as<int32_t>());
         part_off = c.part_off.as<int32_t>();
+    }
+    // quarter-cell part tables of cells with kPartMin+ points (before the
+    // dense re-sort, like part_off)
+    const int32_t *part_fine = nullptr, *big_id = nullptr;
+    if (c.sub_bits >= 2 * D) {
+        c.big_id.ensure((size_t)(nc + 1) * 4);
+        c.gflag.ensure((size_t)(nc + 1) * 4);
+        c.part_fine.ensure((size_t)(n / kPartMin + 1) * 64 * 4);
+        {
+            Launch L(c.prof, s, "part_off");
+            k_big_flag<<<(unsigned)ceil_div(nc, 256), 256, 0, s>>>(c.cell_start.as<int32_t>(), ncp, nc,
+                                                                   c.gflag.as<int32_t>());
+        }
+        scan_exclusive_i32(c.gflag.as<int32_t>(), c.big_id.as<int32_t>(), nc, nullptr, c.scratch, s, c.prof);
+        {
+            Launch L(c.prof, s, "part_off");
+            k_part_fine<D><<<nb, 256, 0, s>>>(keys, c.sub_bits, c.cell_of.as<int32_t>(), c.cell_start.as<int32_t>(),
+                                             c.big_id.as<int32_t>(), n, c.part_fine.as<int32_t>());
+        }
+        part_fine = c.part_fine.as<int32_t>();
+        big_id = c.big_id.as<int32_t>();
     }
     // dense 2-D cells: strip order and tables (may reorder points inside
     // cells); eager runs only, a captured run counts them as ordinary cells
@@ -1381,7 +1454,7 @@ static void grid_classify_d(Ctx &c) {
         k_count_sym<D><<<(unsigned)ceil_div(nct, 4), 256, 0, s>>>(
             c.ctiles.as<int4>(), ntp, c.cell_start.as<int32_t>(), c.cell_of.as<int32_t>(), c.cell_key.as<uint64_t>(),
             ig, c.lin_bits, c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), ncp, c.soa.as<float>(), npad,
-            c.nn_s.as<int32_t>(), stats ? c.tile_tests.as<int64_t>() : nullptr, part_off);
+            c.nn_s.as<int32_t>(), stats ? c.tile_tests.as<int64_t>() : nullptr, part_fine, big_id);
     }
     if (stats) {
         Launch L(c.prof, s, "stats");
