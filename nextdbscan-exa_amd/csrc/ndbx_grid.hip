@@ -17,7 +17,7 @@
 
 namespace ndbx {
 
-constexpr int64_t kBoundsChunk = 1 << 16;
+constexpr int64_t kBoundsChunk = 1 << 13;  // points per bounds workgroup (enough workgroups to fill the chip)
 constexpr int kFineBits = 6;  // 2-D: points of a cell in Morton order of a 64 x 64 sub-grid
 constexpr int kPartMin = 64;  // count kernel: cells with fewer points are taken whole, not by parts
 constexpr int kRunGap = 8;    // symmetric count: candidate runs closer than this are walked as one

@@ -69,13 +69,17 @@ __global__ __launch_bounds__(256) void k_core_points(const uint8_t *__restrict__
     if (v) core_i32[p] = is_core;
     const int32_t c = cell_of[iv];
     int32_t val = is_core ? (int32_t)p : INT_MAX;
-    const int32_t c_first = __shfl(c, 0, 64);
-    if (__ballot(c != c_first) == 0) {
-        for (int o = 32; o >= 1; o >>= 1) val = min(val, __shfl_xor(val, o, 64));
-        if ((threadIdx.x & 63) == 0 && val != INT_MAX && c < nc) atomicMin(cell_core_min + c, val);
-    } else if (is_core && c < nc) {
-        atomicMin(cell_core_min + c, val);
+    // minimum over each run of lanes of one cell (points are sorted by cell):
+    // a segmented scan, then the run's last lane makes the one atomic
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t vo = __shfl_up(val, o, 64), co = __shfl_up(c, o, 64);
+        if (lane >= o && co == c) val = min(val, vo);
     }
+    const int32_t c_next = __shfl_down(c, 1, 64);
+    const bool last = lane == 63 || c_next != c || !(i + 1 < n);
+    if (v && last && val != INT_MAX && c < nc) atomicMin(cell_core_min + c, val);
 }
 
 __global__ __launch_bounds__(256) void k_has_core(const int32_t *__restrict__ cell_core_min, const int32_t *__restrict__ ncp,
