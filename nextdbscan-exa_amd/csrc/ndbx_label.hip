@@ -386,15 +386,28 @@ __global__ __launch_bounds__(256) void k_union_small(const int32_t *__restrict__
     }
 }
 
-// Every core cell's root and the lowest core id of each root's set.
+// Every core cell's root and the lowest core id of each root's set.  A few
+// roots take all the cells, so neighbouring lanes with one root (cells of a
+// row mostly share one) take their minimum first: a segmented scan over runs
+// of equal roots, one atomic per run.
 __global__ __launch_bounds__(256) void k_flatten(const int32_t *__restrict__ cell_has_core,
                                                  const int32_t *__restrict__ cell_core_min, const int32_t *__restrict__ ncp,
                                                  int32_t *__restrict__ parent, int32_t *__restrict__ root_min) {
     const int64_t nc = *ncp;
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc || !cell_has_core[c]) return;
-    const int32_t r = uf_find(parent, (int32_t)c);
-    atomicMin(root_min + r, cell_core_min[c]);
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool v = c < nc && cell_has_core[c];
+    const int32_t r = v ? uf_find(parent, (int32_t)c) : -1;
+    int32_t val = v ? cell_core_min[c] : INT_MAX;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        // (a hop to an equal root outside the run only adds values of the
+        // same root; within a run the scan is complete)
+        const int32_t vo = __shfl_up(val, o, 64), ro = __shfl_up(r, o, 64);
+        if (lane >= o && ro == r) val = min(val, vo);
+    }
+    const int32_t r_next = __shfl_down(r, 1, 64);
+    if (v && (lane == 63 || r_next != r)) atomicMin(root_min + r, val);
 }
 
 __global__ __launch_bounds__(256) void k_core_base(const int32_t *__restrict__ rank, const int32_t *__restrict__ total,
