@@ -726,8 +726,7 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? 3 : 2) void 
     __shared__ bf16x8_t sb_hi[2][32 * ROWU], sb_lo[2][32 * ROWU];
     __shared__ float sb_na[2][32];
     __shared__ int32_t sb_cacc[kColWin];  // column counts of the current window
-    __shared__ int32_t sb_cell[2], sb_cb[2], sb_ci[2], sb_cs[2], sb_edge;
-    __shared__ float sb_rmax[2];
+    __shared__ int32_t sb_edge;
     __shared__ int4 sb_q[WAVES][kBandQ];  // per wave: guard-band pairs not yet in the global queue
     const int64_t gidx = mf_group_of(blockIdx.x, chunk);
     if (gidx >= A.n_groups) return;
@@ -809,7 +808,7 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? 3 : 2) void 
     // (The norm is selected only when published: a select right after its
     // load would wait for every load of the fetch.)
     bf16x8_t pf[UPT];
-    float pf_na = INF, pf_rmax = 0.0f;
+    float pf_na = INF;
     bool pf_ok = false;
     const int64_t n_last = A.n - 1;
     auto fetch = [&]() {
@@ -825,7 +824,6 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? 3 : 2) void 
         const int jn = cb + (int)(threadIdx.x & 31);
         pf_na = A.na[jn < n_last ? jn : n_last];
         pf_ok = jn < ce;
-        pf_rmax = ld_const(A.cell_rmax + co);
     };
     auto publish = [&](int buf) {
 #pragma unroll
@@ -837,13 +835,6 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? 3 : 2) void 
             else if (u < 2 * UNITS) sb_lo[buf][bc * ROWU + kk] = pf[q];
         }
         if (threadIdx.x < 32) sb_na[buf][threadIdx.x] = pf_ok ? pf_na : INF;
-        if (threadIdx.x == 0) {
-            sb_cell[buf] = co;
-            sb_cb[buf] = cb;
-            sb_ci[buf] = ci;
-            sb_cs[buf] = cs;
-            sb_rmax[buf] = pf_rmax;
-        }
     };
     settle();
     if (ci >= c_end) return;  // cannot happen: a cell always takes itself
@@ -855,23 +846,19 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? 3 : 2) void 
     int buf = 0, cur_o = -1;
     float hb = 0.0f;
     for (;;) {
-        // the next block's loads overlap this block's products
+        // this block's place in the sequence (wave-uniform, in scalar
+        // registers), then the next block's loads, which overlap this
+        // block's products
+        const int o = co, bcb = cb, bci = ci, bcs = cs;
         cb += 32;
         settle();
         const bool more = ci < c_end;
         if (more) fetch();
-        // (wave-uniform: the cell radius below is then a scalar load, which
-        // does not wait for the fetch's vector loads)
-        const int o = __builtin_amdgcn_readfirstlane(sb_cell[buf]);
-        const int bcb = __builtin_amdgcn_readfirstlane(sb_cb[buf]);
-        const int bci = __builtin_amdgcn_readfirstlane(sb_ci[buf]);
-        const int bcs = __builtin_amdgcn_readfirstlane(sb_cs[buf]);
         const bool diag = o == cell;
         if (wave_on) {
             if (o != cur_o) {
                 cur_o = o;
-                const float rmax_col = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(sb_rmax[buf])));
-                const double R = fmax((double)rmax_row, (double)rmax_col);
+                const double R = fmax((double)rmax_row, (double)ld_const(A.cell_rmax + o));
                 hb = (float)(A.b1 * R * R + A.b2 * R + A.b3);  // half the band
             }
             // the accumulator starts at -|b|^2/2 - (|a|^2 - e2)/2 and ends at
@@ -959,10 +946,7 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? 3 : 2) void 
         __syncthreads();
         // leaving the window (another cell, the next kColWin columns, or the
         // end): its column counts and pair flag go to global memory
-        const int ncell = __builtin_amdgcn_readfirstlane(sb_cell[buf ^ 1]);
-        const int ncb = __builtin_amdgcn_readfirstlane(sb_cb[buf ^ 1]);
-        const int ncs = __builtin_amdgcn_readfirstlane(sb_cs[buf ^ 1]);
-        if (!more || ncell != o || (ncb - ncs) / kColWin != (bcb - bcs) / kColWin) {
+        if (!more || co != o || (cb - cs) / kColWin != (bcb - bcs) / kColWin) {
             if (!diag) {
                 const int32_t wb = bcs + (bcb - bcs) / kColWin * kColWin;
                 for (int t = threadIdx.x; t < kColWin; t += blockDim.x) {
