@@ -64,7 +64,7 @@ __device__ __forceinline__ int32_t stage_test(const float (&qx)[D], const float 
                     const float df = qx[k] - c[k][u];
                     l2 = __fadd_rn(l2, __fmul_rn(df, df));
                 }
-                cnt += (l2 <= e2) ? 1 : 0;
+                cnt = add_if(cnt, l2 <= e2);
             }
         }
     } else {
@@ -88,7 +88,7 @@ __device__ __forceinline__ int32_t stage_test(const float (&qx)[D], const float 
                     const float df = qx[k] - c[k][u];
                     l2 = __fadd_rn(l2, __fmul_rn(df, df));
                 }
-                cnt += (l2 <= e2) ? 1 : 0;
+                cnt = add_if(cnt, l2 <= e2);
             }
         }
     }

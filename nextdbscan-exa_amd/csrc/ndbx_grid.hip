@@ -621,13 +621,12 @@ __global__ __launch_bounds__(256) void k_group_tile_fill(const int32_t *__restri
 // Unrolled candidate block: U candidates [j, j+U) of a wave-uniform range.
 template <int D, int U>
 __device__ __forceinline__ int32_t count_block(const float (&qx)[D], const float *__restrict__ soa, int64_t npad,
-                                               int j, float e2) {
+                                               int j, float e2, int32_t cnt) {
     float c[D][U];
 #pragma unroll
     for (int k = 0; k < D; ++k)
 #pragma unroll
         for (int u = 0; u < U; ++u) c[k][u] = soa[k * npad + j + u];
-    int32_t cnt = 0;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         float len = 0.0f;
@@ -636,7 +635,7 @@ __device__ __forceinline__ int32_t count_block(const float (&qx)[D], const float
             const float df = qx[k] - c[k][u];
             len = __fadd_rn(len, __fmul_rn(df, df));
         }
-        cnt += (len <= e2) ? 1 : 0;
+        cnt = add_if(cnt, len <= e2);
     }
     return cnt;
 }
@@ -729,8 +728,8 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
                         const int end = cell_start[B];
                         int j = run;
                         issued += end - j;
-                        for (; j + 8 <= end; j += 8) cnt += count_block<D, 8>(qx, soa, npad, j, e2);
-                        for (; j < end; ++j) cnt += count_block<D, 1>(qx, soa, npad, j, e2);
+                        for (; j + 8 <= end; j += 8) cnt = count_block<D, 8>(qx, soa, npad, j, e2, cnt);
+                        for (; j < end; ++j) cnt = count_block<D, 1>(qx, soa, npad, j, e2, cnt);
                         if (B == e) break;
                         cnt += dense_cell_count<1>(qx, lo, hi, g, cell_c[B], cell_c[nc + B], dv.info[di], dv.tbl,
                                                    soa, npad, e2, &stage[wave][0][0], lane, bulk, issued);
@@ -779,8 +778,8 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
                     if (B < e && cell_start[B + 1] - b0 < kPartMin) continue;
                     int j = run;
                     issued += b0 - j;
-                    for (; j + 8 <= b0; j += 8) cnt += count_block<D, 8>(qx, soa, npad, j, e2);
-                    for (; j < b0; ++j) cnt += count_block<D, 1>(qx, soa, npad, j, e2);
+                    for (; j + 8 <= b0; j += 8) cnt = count_block<D, 8>(qx, soa, npad, j, e2, cnt);
+                    for (; j < b0; ++j) cnt = count_block<D, 1>(qx, soa, npad, j, e2, cnt);
                     if (B == e) break;
                     const double X = g.origin[0] + (double)cell_c[B] * s_;
                     double gx[2];
@@ -808,8 +807,8 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
                         int jj = po[o1];
                         const int end = o2 < (1 << D) ? po[o2] : b_end;
                         issued += end - jj;
-                        for (; jj + 8 <= end; jj += 8) cnt += count_block<D, 8>(qx, soa, npad, jj, e2);
-                        for (; jj < end; ++jj) cnt += count_block<D, 1>(qx, soa, npad, jj, e2);
+                        for (; jj + 8 <= end; jj += 8) cnt = count_block<D, 8>(qx, soa, npad, jj, e2, cnt);
+                        for (; jj < end; ++jj) cnt = count_block<D, 1>(qx, soa, npad, jj, e2, cnt);
                     }
                     run = b_end;
                 }
@@ -818,8 +817,8 @@ __global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, 
             const int end = __builtin_amdgcn_readlane(pe, slot);
             int j = __builtin_amdgcn_readlane(ps, slot);
             issued += end - j;
-            for (; j + 8 <= end; j += 8) cnt += count_block<D, 8>(qx, soa, npad, j, e2);
-            for (; j < end; ++j) cnt += count_block<D, 1>(qx, soa, npad, j, e2);
+            for (; j + 8 <= end; j += 8) cnt = count_block<D, 8>(qx, soa, npad, j, e2, cnt);
+            for (; j < end; ++j) cnt = count_block<D, 1>(qx, soa, npad, j, e2, cnt);
         }
         if (valid) nn_s[q] = cnt;
         if (tests && lane == 0) tests[t] = issued * tile.w;
@@ -914,13 +913,12 @@ __device__ __forceinline__ uint32_t pack16(uint32_t a, uint32_t b) {
 template <int D, int U, bool MASKED>
 __device__ __forceinline__ int32_t count_block_sym(const float (&qx)[D], const float *__restrict__ soa, int64_t npad,
                                                    int j, float e2, int my_end, int jb, int lane, uint32_t sh,
-                                                   int32_t &cacc) {
+                                                   int32_t cnt, int32_t &cacc) {
     float c[D][U];
 #pragma unroll
     for (int k = 0; k < D; ++k)
 #pragma unroll
         for (int u = 0; u < U; ++u) c[k][u] = soa[k * npad + j + u];
-    int32_t cnt = 0;
     uint32_t bc[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -932,7 +930,7 @@ __device__ __forceinline__ int32_t count_block_sym(const float (&qx)[D], const f
         }
         const float lim = MASKED ? (j + u >= my_end ? e2 : -1.0f) : e2;
         const bool ok = len <= lim;
-        cnt += ok ? 1 : 0;
+        cnt = add_if(cnt, ok);
         bc[u] = (uint32_t)__builtin_popcountll(__ballot(ok));
     }
     const int base = j - jb;
@@ -953,15 +951,14 @@ __device__ __forceinline__ int32_t count_block_sym(const float (&qx)[D], const f
 template <int D, bool MASKED>
 __device__ __forceinline__ int32_t count_run_sym(const float (&qx)[D], const float *__restrict__ soa, int64_t npad,
                                                  int j0, int end, float e2, int my_end, int lane,
-                                                 int32_t *__restrict__ nn_s) {
-    int32_t cnt = 0;
+                                                 int32_t *__restrict__ nn_s, int32_t cnt) {
     const uint32_t sh = 8u * ((lane & 4) | (lane & 1) << 1 | (lane & 2) >> 1);
     for (int jb = j0; jb < end; jb += 64) {
         const int je = end - jb < 64 ? end : jb + 64;
         int32_t cacc = 0;
         int j = jb;
-        for (; j + 8 <= je; j += 8) cnt += count_block_sym<D, 8, MASKED>(qx, soa, npad, j, e2, my_end, jb, lane, sh, cacc);
-        for (; j < je; ++j) cnt += count_block_sym<D, 1, MASKED>(qx, soa, npad, j, e2, my_end, jb, lane, sh, cacc);
+        for (; j + 8 <= je; j += 8) cnt = count_block_sym<D, 8, MASKED>(qx, soa, npad, j, e2, my_end, jb, lane, sh, cnt, cacc);
+        for (; j < je; ++j) cnt = count_block_sym<D, 1, MASKED>(qx, soa, npad, j, e2, my_end, jb, lane, sh, cnt, cacc);
         if (lane < je - jb && cacc != 0) atomicAdd(nn_s + jb + lane, cacc);
     }
     return cnt;
@@ -1025,7 +1022,7 @@ __global__ __launch_bounds__(256) void k_count_sym(const int4 *__restrict__ ctil
 #pragma unroll
         for (int k = 0; k < D; ++k) qw[k] = valid ? qx[k] : __builtin_inff();
         issued += m_end - m_beg;
-        cnt += count_run_sym<D, true>(qw, soa, npad, m_beg, m_end, e2, my_end, lane, nn_s);
+        cnt = count_run_sym<D, true>(qw, soa, npad, m_beg, m_end, e2, my_end, lane, nn_s, cnt);
         uint64_t rows = __ballot(pe > ps);
         const double r2o = search_r2(e2);
         bool small_box = part_fine != nullptr;
@@ -1055,7 +1052,7 @@ __global__ __launch_bounds__(256) void k_count_sym(const int4 *__restrict__ ctil
                     // cells with few points are taken whole (merged into the run)
                     if (B < e && cell_start[B + 1] - b0 < kPartMin) continue;
                     issued += b0 - run;
-                    cnt += count_run_sym<D, false>(qw, soa, npad, run, b0, e2, my_end, lane, nn_s);
+                    cnt = count_run_sym<D, false>(qw, soa, npad, run, b0, e2, my_end, lane, nn_s, cnt);
                     if (B == e) break;
                     const double a0 = g.origin[0] + (double)cell_c[B] * s_ + (double)fine_q<D>(lane, 0) * qs - slack;
                     const double a1 = a0 + qs + 2.0 * slack;
@@ -1080,14 +1077,14 @@ __global__ __launch_bounds__(256) void k_count_sym(const int4 *__restrict__ ctil
                         }
                         if (pj >= 0) {
                             issued += pe - pj;
-                            cnt += count_run_sym<D, false>(qw, soa, npad, pj, pe, e2, my_end, lane, nn_s);
+                            cnt = count_run_sym<D, false>(qw, soa, npad, pj, pe, e2, my_end, lane, nn_s, cnt);
                         }
                         pj = jj;
                         pe = end;
                     }
                     if (pj >= 0) {
                         issued += pe - pj;
-                        cnt += count_run_sym<D, false>(qw, soa, npad, pj, pe, e2, my_end, lane, nn_s);
+                        cnt = count_run_sym<D, false>(qw, soa, npad, pj, pe, e2, my_end, lane, nn_s, cnt);
                     }
                     run = b_end;
                 }
@@ -1096,7 +1093,7 @@ __global__ __launch_bounds__(256) void k_count_sym(const int4 *__restrict__ ctil
             const int end = __builtin_amdgcn_readlane(pe, slot);
             const int j = __builtin_amdgcn_readlane(ps, slot);
             issued += end - j;
-            cnt += count_run_sym<D, false>(qw, soa, npad, j, end, e2, my_end, lane, nn_s);
+            cnt = count_run_sym<D, false>(qw, soa, npad, j, end, e2, my_end, lane, nn_s, cnt);
         }
         issued *= tile.w;
     } else {
@@ -1145,7 +1142,7 @@ __global__ __launch_bounds__(256) void k_count_sym(const int4 *__restrict__ ctil
                         l2 = __fadd_rn(l2, __fmul_rn(df, df));
                     }
                     const bool ok = okc && l2 <= e2 && j >= __builtin_amdgcn_readlane(my_end, i);
-                    ccount += ok ? 1 : 0;
+                    ccount = add_if(ccount, ok);
                     qcnt[i] += __builtin_popcountll(__ballot(ok));
                 }
             }

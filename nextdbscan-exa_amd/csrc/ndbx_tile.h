@@ -18,6 +18,18 @@ __device__ __forceinline__ float wave_max(float v) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
+// cnt + (ok ? 1 : 0) as one v_addc_co_u32 whose carry-in is the compare's
+// lane mask (the mask the ballot reads).  Written plainly, each compare is
+// widened to 0/1 by a v_cndmask before the adds: 1.5 VALU per test instead
+// of 1.
+__device__ __forceinline__ int32_t add_if(int32_t cnt, bool ok) {
+    const uint64_t m = __ballot(ok);
+    int32_t r;
+    uint64_t carry;
+    asm("v_addc_co_u32_e64 %0, %1, 0, %2, %3" : "=v"(r), "=s"(carry) : "v"(cnt), "s"(m));
+    return r;
+}
+
 // Tests the staged candidates [0, n) of a wave's LDS stage (kStage slots per
 // dimension; slots past n hold +inf, which no query accepts) for lanes split
 // as (query lane & (2^qb - 1), slice lane >> qb): slice s takes the groups of
@@ -45,7 +57,7 @@ __device__ __forceinline__ int32_t stage_test_sliced(const float (&qx)[D], const
                 const float df = qx[k] - c[k][u];
                 l2 = __fadd_rn(l2, __fmul_rn(df, df));
             }
-            cnt += (l2 <= e2) ? 1 : 0;
+            cnt = add_if(cnt, l2 <= e2);
         }
     }
     return cnt;
