@@ -82,11 +82,21 @@ __global__ __launch_bounds__(256) void k_core_points(const uint8_t *__restrict__
     if (v && last && val != INT_MAX && c < nc) atomicMin(cell_core_min + c, val);
 }
 
+// Core cells as int32 flags and as a bitmask (bit c & 63 of word c >> 6): the
+// union passes test the bits of a row's cells with one or two loads, and the
+// mask (nc / 8 bytes) stays in L2 where the flags would not.
 __global__ __launch_bounds__(256) void k_has_core(const int32_t *__restrict__ cell_core_min, const int32_t *__restrict__ ncp,
-                                                  int32_t *__restrict__ cell_has_core) {
+                                                  int32_t *__restrict__ cell_has_core, uint64_t *__restrict__ core_bits) {
     const int64_t nc = *ncp;
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < nc) cell_has_core[c] = cell_core_min[c] != INT_MAX;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // a wave starts at a multiple of 64
+    const bool h = c < nc && cell_core_min[c] != INT_MAX;
+    if (c < nc) cell_has_core[c] = h;
+    const uint64_t m = __ballot(h);
+    if ((threadIdx.x & 63) == 0 && c < nc) core_bits[c >> 6] = m;
+}
+
+__device__ __forceinline__ bool core_cell(const uint64_t *__restrict__ core_bits, int c) {
+    return (core_bits[c >> 6] >> (c & 63)) & 1;
 }
 
 __global__ __launch_bounds__(256) void k_iota(int32_t *__restrict__ p, int64_t n) {
@@ -94,27 +104,48 @@ __global__ __launch_bounds__(256) void k_iota(int32_t *__restrict__ p, int64_t n
     if (i < n) p[i] = (int32_t)i;
 }
 
-// Core-only copy of the sorted coordinates: a non-core point becomes NaN in
+// Core-only copy of the sorted coordinates, one point after another
+// (kCoreStride floats each: a 16-byte slot in 3-D), so the few points of a
+// small cell are one or two cache lines: a non-core point becomes NaN in
 // every dimension, so the float32 predicate rejects it without a flag test.
 template <int D>
-__global__ __launch_bounds__(256) void k_soa_core(const float *__restrict__ soa, const uint8_t *__restrict__ core_s,
-                                                  int64_t n, int64_t npad, float *__restrict__ soa_core) {
+constexpr int kCoreStride = D == 3 ? 4 : D;
+
+template <int D>
+__device__ __forceinline__ void load_core(const float *__restrict__ pts, int64_t i, float (&p)[D]) {
+    if constexpr (D == 3) {
+        const float4 v = reinterpret_cast<const float4 *>(pts)[i];
+        p[0] = v.x, p[1] = v.y, p[2] = v.z;
+    } else if constexpr (D == 2) {
+        const float2 v = reinterpret_cast<const float2 *>(pts)[i];
+        p[0] = v.x, p[1] = v.y;
+    } else {
+        p[0] = pts[i];
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_core_pts(const float *__restrict__ soa, const uint8_t *__restrict__ core_s,
+                                                  int64_t n, int64_t npad, float *__restrict__ core_pts) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const bool c = core_s[i];
+    float p[kCoreStride<D>];
 #pragma unroll
-    for (int k = 0; k < D; ++k) soa_core[k * npad + i] = c ? soa[k * npad + i] : __builtin_nanf("");
+    for (int k = 0; k < kCoreStride<D>; ++k) p[k] = c && k < D ? soa[k * npad + i] : __builtin_nanf("");
+    if constexpr (D == 3) reinterpret_cast<float4 *>(core_pts)[i] = make_float4(p[0], p[1], p[2], p[3]);
+    else if constexpr (D == 2) reinterpret_cast<float2 *>(core_pts)[i] = make_float2(p[0], p[1]);
+    else core_pts[i] = p[0];
 }
 
 // Any of U candidates [j, j+U) within eps of this lane's query.
 template <int D, int U>
-__device__ __forceinline__ bool any_block(const float (&qx)[D], const float *__restrict__ soa, int64_t npad, int j,
-                                          float e2) {
+__device__ __forceinline__ bool any_block(const float (&qx)[D], const float *__restrict__ pts, int j, float e2) {
     float c[D][U];
 #pragma unroll
     for (int k = 0; k < D; ++k)
 #pragma unroll
-        for (int u = 0; u < U; ++u) c[k][u] = soa[k * npad + j + u];
+        for (int u = 0; u < U; ++u) c[k][u] = pts[(int64_t)(j + u) * kCoreStride<D> + k];
     bool hit = false;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -131,13 +162,13 @@ __device__ __forceinline__ bool any_block(const float (&qx)[D], const float *__r
 
 // True when some lane's query has a candidate in [b, e) within eps.
 template <int D>
-__device__ __forceinline__ bool wave_any_within(const float (&qx)[D], const float *__restrict__ soa, int64_t npad,
-                                                int b, int e, float e2) {
+__device__ __forceinline__ bool wave_any_within(const float (&qx)[D], const float *__restrict__ pts, int b, int e,
+                                                float e2) {
     int j = b;
     for (; j + 8 <= e; j += 8)
-        if (__ballot(any_block<D, 8>(qx, soa, npad, j, e2)) != 0) return true;
+        if (__ballot(any_block<D, 8>(qx, pts, j, e2)) != 0) return true;
     for (; j < e; ++j)
-        if (__ballot(any_block<D, 1>(qx, soa, npad, j, e2)) != 0) return true;
+        if (__ballot(any_block<D, 1>(qx, pts, j, e2)) != 0) return true;
     return false;
 }
 
@@ -160,8 +191,8 @@ __global__ __launch_bounds__(256) void k_union(const int32_t *__restrict__ list,
                                                const InstGrid *__restrict__ ig, int lin_bits,
                                                const int2 *__restrict__ ranges, int rows_per_cell,
                                                const int32_t *__restrict__ cell_c,
-                                               const float *__restrict__ soa_core, int64_t npad,
-                                               const int32_t *__restrict__ cell_has_core,
+                                               const float *__restrict__ core_pts,
+                                               const uint64_t *__restrict__ core_bits,
                                                int32_t *__restrict__ parent) {
     const int64_t nc = *ncp;
     __shared__ int32_t cand[4][kMaxStencilCells];
@@ -189,7 +220,7 @@ __global__ __launch_bounds__(256) void k_union(const int32_t *__restrict__ list,
             const int want0 = lane == 0 ? ac0 + 1 : ac0;
             for (int b = w.x; b < w.y; ++b)
                 if (cell_c[b] == want0) B = b;
-            if (B >= 0 && !cell_has_core[B]) B = -1;
+            if (B >= 0 && !core_cell(core_bits, B)) B = -1;
         }
         const uint64_t mask = __ballot(B >= 0);
         if (B >= 0) cand[wave][__popcll(mask & lt)] = B;
@@ -203,7 +234,7 @@ __global__ __launch_bounds__(256) void k_union(const int32_t *__restrict__ list,
 #pragma unroll
         for (int it = 0; it < 5; ++it) {
             const int B = my.x + it;
-            const bool keep = B < my.y && B > A && cell_has_core[B];
+            const bool keep = B < my.y && B > A && core_cell(core_bits, B);
             Bs[it] = keep ? B : -1;
             int cl = 0;
             if (keep) {
@@ -254,8 +285,7 @@ __global__ __launch_bounds__(256) void k_union(const int32_t *__restrict__ list,
     for (int q0 = a0; q0 < a1; q0 += 64) {
         const int q = q0 + lane;
         float qx[D], lo[D], hi[D];
-#pragma unroll
-        for (int k = 0; k < D; ++k) qx[k] = soa_core[k * npad + (q < a1 ? q : a0)];
+        load_core<D>(core_pts, q < a1 ? q : a0, qx);
         if (q >= a1) qx[0] = __builtin_nanf("");
         const bool is_core = qx[0] == qx[0];
         if (__ballot(is_core) == 0) continue;
@@ -282,7 +312,7 @@ __global__ __launch_bounds__(256) void k_union(const int32_t *__restrict__ list,
                 gap2 += gk * gk;
             }
             if (__builtin_amdgcn_readfirstlane(gap2 > r2)) continue;
-            if (wave_any_within<D>(qx, soa_core, npad, cell_start[B], cell_start[B + 1], e2)) {
+            if (wave_any_within<D>(qx, core_pts, cell_start[B], cell_start[B + 1], e2)) {
                 if (lane == 0) {
                     cand[wave][ci] = -1;
                     uf_unite(parent, A, B);
@@ -333,7 +363,12 @@ __global__ __launch_bounds__(256) void k_cell_lists(const int32_t *__restrict__ 
 
 // One lane per small core cell A: every stencil neighbour B > A with a core
 // and a different root is tested pair by pair (its cores against A's cores,
-// non-cores are NaN in soa_core) until a pair within eps unites them.
+// non-cores are NaN in core_pts) until a pair within eps unites them.  Rows
+// before A's own row (slots below the centre) hold only cells below A.  The
+// cells of a row window are consecutive, so their core bits come from one or
+// two mask words.  A's root is kept between candidates and found again when
+// a candidate's root differs from it (sets only merge, so equal roots at any
+// time mean one set).
 template <int D>
 __global__ __launch_bounds__(256) void k_union_small(const int32_t *__restrict__ list,
                                                      const int32_t *__restrict__ n_list,
@@ -341,8 +376,8 @@ __global__ __launch_bounds__(256) void k_union_small(const int32_t *__restrict__
                                                      const uint64_t *__restrict__ cell_key,
                                                      const InstGrid *__restrict__ ig, int lin_bits,
                                                      const int2 *__restrict__ ranges, int rows_per_cell,
-                                                     const float *__restrict__ soa_core, int64_t npad,
-                                                     const int32_t *__restrict__ cell_has_core,
+                                                     const float *__restrict__ core_pts,
+                                                     const uint64_t *__restrict__ core_bits,
                                                      int32_t *__restrict__ parent) {
     const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (li >= *n_list) return;
@@ -353,24 +388,31 @@ __global__ __launch_bounds__(256) void k_union_small(const int32_t *__restrict__
     int na = 0;
     for (int i = a0; i < a1; ++i) {
         float p[D];
-#pragma unroll
-        for (int k = 0; k < D; ++k) p[k] = soa_core[k * npad + i];
+        load_core<D>(core_pts, i, p);
         if (p[0] != p[0]) continue;  // not a core
 #pragma unroll
         for (int k = 0; k < D; ++k) ax[na][k] = p[k];
         ++na;
     }
-    for (int slot = 0; slot < rows_per_cell; ++slot) {
+    int32_t ra = uf_find(parent, A);
+    for (int slot = rows_per_cell / 2; slot < rows_per_cell; ++slot) {
         const int2 w = ranges[(int64_t)A * rows_per_cell + slot];
-        for (int B = w.x > A ? w.x : A + 1; B < w.y; ++B) {
-            if (!cell_has_core[B]) continue;
-            if (uf_find(parent, A) == uf_find(parent, B)) continue;
+        const int b0 = w.x > A ? w.x : A + 1;
+        if (b0 >= w.y) continue;
+        // core bits of the window's cells [b0, w.y) (at most 2 reach + 1 <= 5 cells)
+        const int sh = b0 & 63;
+        uint64_t bits = core_bits[b0 >> 6] >> sh;
+        if (sh != 0 && ((w.y - 1) >> 6) != (b0 >> 6)) bits |= core_bits[(b0 >> 6) + 1] << (64 - sh);
+        for (int B = b0; B < w.y; ++B, bits >>= 1) {
+            if (!(bits & 1)) continue;
+            const int32_t rb = uf_find(parent, B);
+            if (rb != ra) ra = uf_find(parent, A);  // A's set may have been hung under another since
+            if (rb == ra) continue;
             const int b1 = cell_start[B + 1];
             bool hit = false;
             for (int j = cell_start[B]; j < b1 && !hit; ++j) {
                 float p[D];
-#pragma unroll
-                for (int k = 0; k < D; ++k) p[k] = soa_core[k * npad + j];
+                load_core<D>(core_pts, j, p);
                 for (int i = 0; i < na && !hit; ++i) {
                     float len = 0.0f;
 #pragma unroll
@@ -454,7 +496,7 @@ __global__ __launch_bounds__(256) void k_border(const int4 *__restrict__ ctiles,
                                                 const int2 *__restrict__ ranges, int rows_per_cell,
                                                 const int32_t *__restrict__ cell_c, const int32_t *__restrict__ ncp,
                                                 const int32_t *__restrict__ cell_of,
-                                                const float *__restrict__ soa, const float *__restrict__ soa_core,
+                                                const float *__restrict__ soa, const float *__restrict__ core_pts,
                                                 int64_t npad, const uint8_t *__restrict__ core_s,
                                                 const int32_t *__restrict__ nn_s,
                                                 const int32_t *__restrict__ cell_label,
@@ -502,11 +544,11 @@ __global__ __launch_bounds__(256) void k_border(const int4 *__restrict__ ctiles,
                 int j = cell_start[B];
                 bool open = true;  // some wanting lane has no hit yet (wave-uniform)
                 for (; j + 8 <= b1 && open; j += 8) {
-                    got |= any_block<D, 8>(qw, soa_core, npad, j, e2);
+                    got |= any_block<D, 8>(qw, core_pts, j, e2);
                     open = __ballot(want && !got) != 0;
                 }
                 for (; j < b1 && open; ++j) {
-                    got |= any_block<D, 1>(qw, soa_core, npad, j, e2);
+                    got |= any_block<D, 1>(qw, core_pts, j, e2);
                     open = __ballot(want && !got) != 0;
                 }
                 if (got) best = LB;
@@ -604,6 +646,7 @@ static void grid_label_d(Ctx &c) {
 
     c.cell_core_min.ensure(nc * 4);
     c.cell_has_core.ensure(nc * 4);
+    c.core_bits.ensure((size_t)ceil_div(nc, 64) * 8);
     c.rank.ensure((n + 1) * 4);
     c.label_s.ensure(n * 4);  // core flags as int32, input order
     fill_i32(c.cell_core_min.as<int32_t>(), INT_MAX, nc, s, c.prof);
@@ -611,7 +654,8 @@ static void grid_label_d(Ctx &c) {
         Launch L(c.prof, s, "core");
         k_core_points<<<nb, 256, 0, s>>>(c.core_s.as<uint8_t>(), c.vals.as<uint32_t>(), c.cell_of.as<int32_t>(), n,
                                          ncp, c.label_s.as<int32_t>(), c.cell_core_min.as<int32_t>());
-        k_has_core<<<ncb, 256, 0, s>>>(c.cell_core_min.as<int32_t>(), ncp, c.cell_has_core.as<int32_t>());
+        k_has_core<<<ncb, 256, 0, s>>>(c.cell_core_min.as<int32_t>(), ncp, c.cell_has_core.as<int32_t>(),
+                                       c.core_bits.as<uint64_t>());
     }
     int32_t *d_total = c.misc.as<int32_t>();
     scan_exclusive_i32(c.label_s.as<int32_t>(), c.rank.as<int32_t>(), n, d_total, c.scratch, s, c.prof);
@@ -627,10 +671,10 @@ static void grid_label_d(Ctx &c) {
         k_iota<<<ncb, 256, 0, s>>>(c.parent.as<int32_t>(), nc);
     }
     fill_i32(c.comp_min.as<int32_t>(), INT_MAX, nc, s, c.prof);
-    c.soa_core.ensure((size_t)npad * D * 4);
+    c.soa_core.ensure((size_t)n * kCoreStride<D> * 4);
     {
-        Launch L(c.prof, s, "soa_core");
-        k_soa_core<D><<<nb, 256, 0, s>>>(c.soa.as<float>(), c.core_s.as<uint8_t>(), n, npad, c.soa_core.as<float>());
+        Launch L(c.prof, s, "core_pts");
+        k_core_pts<D><<<nb, 256, 0, s>>>(c.soa.as<float>(), c.core_s.as<uint8_t>(), n, npad, c.soa_core.as<float>());
     }
     // compacted core-cell lists (big: wave per cell; small: lane per cell)
     c.gflag.ensure(nc * 4);
@@ -648,14 +692,14 @@ static void grid_label_d(Ctx &c) {
         k_union<D><<<(unsigned)ceil_div(nc, 4), 256, 0, s>>>(big, counts, ncp, phase, c.cell_start.as<int32_t>(),
                                                              c.cell_key.as<uint64_t>(), ig, c.lin_bits,
                                                              c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(),
-                                                             c.soa_core.as<float>(), npad,
-                                                             c.cell_has_core.as<int32_t>(), c.parent.as<int32_t>());
+                                                             c.soa_core.as<float>(), c.core_bits.as<uint64_t>(),
+                                                             c.parent.as<int32_t>());
     }
     {
         Launch L(c.prof, s, "union_small");
         k_union_small<D><<<ncb, 256, 0, s>>>(small, counts + 1, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(),
-                                             ig, c.lin_bits, c.ranges.as<int2>(), rows, c.soa_core.as<float>(), npad,
-                                             c.cell_has_core.as<int32_t>(), c.parent.as<int32_t>());
+                                             ig, c.lin_bits, c.ranges.as<int2>(), rows, c.soa_core.as<float>(),
+                                             c.core_bits.as<uint64_t>(), c.parent.as<int32_t>());
     }
     {
         Launch L(c.prof, s, "flatten");
