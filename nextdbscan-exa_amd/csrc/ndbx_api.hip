@@ -25,7 +25,7 @@ Ctx::~Ctx() {
                       &nranges, &ranges, &tile_off, &tiles, &nn_s, &core_s, &cell_core_min, &cell_has_core,
                       &parent, &comp_min, &cell_label, &label_s, &rank, &misc, &scratch, &scratch2, &b_parent,
                       &b_rootmin, &soa_core, &cell_c, &gflag, &ctile_off, &ctiles, &tile_tests, &dense_id, &dense_info, &dense_tmp, &fine_off, &dcount, &plan_in, &part_off,
-                      &pflag, &bq, &cgroups, &part_fine, &big_id, &core_bits};
+                      &pflag, &bq, &cgroups, &part_fine, &big_id, &core_bits, &border_tiles};
     for (auto *b : bufs) b->release();
     if (stream) (void)hipStreamDestroy(stream);
 }

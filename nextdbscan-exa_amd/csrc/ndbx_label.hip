@@ -482,14 +482,146 @@ __global__ __launch_bounds__(256) void k_cell_label(const int32_t *__restrict__ 
 
 // Labels per point: cores take their cell's id; a non-core point takes the
 // smallest id among core cells holding a core within eps (its own cell's
-// cores are within eps by construction).
-// One wave per count tile.  Cores keep their cell's id.  A non-core point
-// starts from its own cell's id (the cores of its cell are within eps by
-// construction) and takes the id of any core cell B with a smaller id that
-// holds a core within eps.  Windows are the count windows of the tile, over
-// the box of its non-core points.
+// cores are within eps by construction).  Three launches:
+//   k_border_own   one lane per point: its own cell's id, in input order;
+//   k_border_list  one lane per count tile: non-core points with a
+//                  neighbour besides themselves (nn > 1; with nn = 1 a point
+//                  is within eps of no core) search; tiles of more than
+//                  kBorderLaneTile points that hold one are listed for a wave
+//                  each, the searching points of smaller tiles for a lane
+//                  each;
+//   k_border       waves over the listed tiles: the searching points look
+//                  for a core cell with a smaller id holding a core within
+//                  eps, the candidates broadcast to the wave;
+//   k_border_pts   one lane per listed point, the same search alone: sparse
+//                  tiles would leave most of a wave idle, and a wave has many
+//                  more lane searches than tile searches in flight.
+constexpr int kBorderLaneTile = 16;
+__global__ __launch_bounds__(256) void k_border_own(const int32_t *__restrict__ cell_of,
+                                                    const int32_t *__restrict__ cell_label,
+                                                    const uint32_t *__restrict__ perm, int64_t n,
+                                                    int32_t *__restrict__ labels) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) labels[perm[i]] = cell_label[cell_of[i]];  // NO_CLUSTER when the cell has no core
+}
+
+__global__ __launch_bounds__(256) void k_border_list(const int4 *__restrict__ ctiles, const int32_t *__restrict__ ntp,
+                                                     const uint8_t *__restrict__ core_s,
+                                                     const int32_t *__restrict__ nn_s, int32_t *__restrict__ list,
+                                                     int32_t *__restrict__ n_list, int32_t *__restrict__ plist,
+                                                     int32_t *__restrict__ n_plist) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool wave_tile = false;
+    int np = 0;  // searching points of a small tile
+    int4 tile = make_int4(0, 0, 0, 0);
+    if (t < *ntp) {
+        tile = ctiles[t];
+        if (tile.w > kBorderLaneTile) {
+            for (int q = tile.z; q < tile.z + tile.w && !wave_tile; ++q) wave_tile = !core_s[q] && nn_s[q] > 1;
+        } else {
+            for (int q = tile.z; q < tile.z + tile.w; ++q) np += !core_s[q] && nn_s[q] > 1;
+        }
+    }
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t m = __ballot(wave_tile);
+    if (m != 0) {
+        int32_t base = 0;
+        if (lane == 0) base = atomicAdd(n_list, __popcll(m));
+        base = __shfl(base, 0, 64);
+        if (wave_tile) list[base + __popcll(m & lt)] = (int32_t)t;
+    }
+    if (__ballot(np != 0) != 0) {
+        int incl = np;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t2 = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t2;
+        }
+        int32_t base = 0;
+        if (lane == 63) base = atomicAdd(n_plist, incl);
+        base = __shfl(base, 63, 64);
+        int pos = base + incl - np;
+        if (np)
+            for (int q = tile.z; q < tile.z + tile.w; ++q)
+                if (!core_s[q] && nn_s[q] > 1) plist[pos++] = q;
+    }
+}
+
+// One lane per listed point q (sorted order): the rows of its cell's stencil
+// that some point within the search radius can lie in, and in them the core
+// cells with an id below the point's current one, tested point by point.
 template <int D>
-__global__ __launch_bounds__(256) void k_border(const int4 *__restrict__ ctiles, const int32_t *__restrict__ ntp,
+__global__ __launch_bounds__(256) void k_border_pts(const int32_t *__restrict__ plist, const int32_t *__restrict__ n_plist,
+                                                    const int32_t *__restrict__ cell_start,
+                                                    const uint64_t *__restrict__ cell_key,
+                                                    const InstGrid *__restrict__ ig, int lin_bits,
+                                                    const int2 *__restrict__ ranges, int rows_per_cell,
+                                                    const int32_t *__restrict__ cell_c, const int32_t *__restrict__ ncp,
+                                                    const int32_t *__restrict__ cell_of,
+                                                    const float *__restrict__ soa, const float *__restrict__ core_pts,
+                                                    int64_t npad, const int32_t *__restrict__ cell_label,
+                                                    const uint32_t *__restrict__ perm, int32_t *__restrict__ labels) {
+    const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= *n_plist) return;
+    const int64_t nc = *ncp;
+    const int q = plist[li];
+    const int A = cell_of[q];
+    const int32_t own = cell_label[A];
+    int32_t best = own;
+    const InstGrid &g = ig[cell_key[A] >> lin_bits];
+    const float e2 = g.e2;
+    float qx[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) qx[k] = soa[k * npad + q];
+    const double inv_s = g.inv_s, s = 1.0 / inv_s, slack = s * 9.5367431640625e-07;  // 2^-20 cell
+    const double r2 = search_r2(e2);
+    const int c1 = D >= 2 ? cell_c[nc + A] : 0, c2 = D >= 3 ? cell_c[2 * nc + A] : 0;
+    for (int slot = 0; slot < rows_per_cell; ++slot) {
+        double rem = r2;
+        if (D >= 2) {
+            const int dy = (D == 3 ? slot % 5 : slot) - 2;
+            const double ylo = g.origin[1] + (double)(c1 + dy) * s - slack, yhi = ylo + s + 2.0 * slack;
+            const double gy = fmax(0.0, fmax(ylo - (double)qx[1], (double)qx[1] - yhi));
+            rem -= gy * gy;
+        }
+        if (D >= 3) {
+            const int dz = slot / 5 - 2;
+            const double zlo = g.origin[2] + (double)(c2 + dz) * s - slack, zhi = zlo + s + 2.0 * slack;
+            const double gz = fmax(0.0, fmax(zlo - (double)qx[2], (double)qx[2] - zhi));
+            rem -= gz * gz;
+        }
+        if (rem < 0.0) continue;
+        const int2 w = ranges[(int64_t)A * rows_per_cell + slot];
+        for (int B = w.x; B < w.y; ++B) {
+            const int32_t LB = cell_label[B];
+            if (LB >= best) continue;  // NO_CLUSTER (no core) included
+            const int b1 = cell_start[B + 1];
+            bool hit = false;
+            for (int j = cell_start[B]; j < b1 && !hit; ++j) {
+                float p[D];
+                load_core<D>(core_pts, j, p);
+                float len = 0.0f;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const float df = qx[k] - p[k];
+                    len = __fadd_rn(len, __fmul_rn(df, df));
+                }
+                hit = len <= e2;
+            }
+            if (hit) best = LB;
+        }
+    }
+    if (best != own) labels[perm[q]] = best;
+}
+
+// A non-core point of a listed tile starts from its own cell's id and takes
+// the id of any core cell B with a smaller id that holds a core within eps.
+// Windows are the count windows of the tile, over the box of its searching
+// points.  Each wave takes listed tiles in a grid-stride loop.
+template <int D>
+__global__ __launch_bounds__(256) void k_border(const int4 *__restrict__ ctiles, const int32_t *__restrict__ list,
+                                                const int32_t *__restrict__ n_list,
                                                 const int32_t *__restrict__ cell_start,
                                                 const uint64_t *__restrict__ cell_key,
                                                 const InstGrid *__restrict__ ig, int lin_bits,
@@ -502,60 +634,86 @@ __global__ __launch_bounds__(256) void k_border(const int4 *__restrict__ ctiles,
                                                 const int32_t *__restrict__ cell_label,
                                                 const uint32_t *__restrict__ perm, int32_t *__restrict__ labels) {
     const int64_t nc = *ncp;
-    const int64_t n_tiles = *ntp;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
-    if (t >= n_tiles) return;
+    const int64_t nl = *n_list;
     const int lane = threadIdx.x & 63;
-    const int4 tile = ctiles[t];
-    const bool valid = lane < tile.w;
-    const int q = tile.z + (valid ? lane : 0);
-    const bool is_core = valid && core_s[q];
-    int32_t best = cell_label[cell_of[q]];  // NO_CLUSTER when the cell has no core
-    // a non-core point with no neighbour but itself (nn = 1) is within eps of no core
-    const bool need = valid && !is_core && nn_s[q] > 1;
-    if (__ballot(need) != 0) {
-        const InstGrid &g = ig[cell_key[tile.x] >> lin_bits];
-        const float e2 = g.e2;
-        float qx[D], lo[D], hi[D];
+    const int64_t wave0 = (int64_t)blockIdx.x * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int64_t li = wave0; li < nl; li += (int64_t)gridDim.x * (blockDim.x >> 6)) {
+        const int4 tile = ctiles[list[li]];
+        const bool valid = lane < tile.w;
+        const int q = tile.z + (valid ? lane : 0);
+        const bool is_core = valid && core_s[q];
+        const int32_t own = cell_label[cell_of[q]];
+        int32_t best = own;
+        const bool need = valid && !is_core && nn_s[q] > 1;
+        if (__ballot(need) != 0) {
+            const InstGrid &g = ig[cell_key[tile.x] >> lin_bits];
+            const float e2 = g.e2;
+            float qx[D], lo[D], hi[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-            qx[k] = soa[k * npad + q];
-            lo[k] = wave_min(need ? qx[k] : __builtin_inff());
-            hi[k] = wave_max(need ? qx[k] : -__builtin_inff());
-        }
-        const int2 w = tile_window<D>(lane, rows_per_cell, tile.x, tile.y, lo, hi, g, cell_c[nc + tile.x],
-                                      cell_c[2 * nc + tile.x], ranges, cell_c, nc);
-        uint64_t rows = __ballot(w.x < w.y);
-        while (rows) {
-            const int slot = __builtin_ctzll(rows);
-            rows &= rows - 1;
-            const int b_end = __builtin_amdgcn_readlane(w.y, slot);
-            for (int B = __builtin_amdgcn_readlane(w.x, slot); B < b_end; ++B) {
-                const int32_t LB = cell_label[B];
-                // only lanes that would improve take part; others see NaN
-                const bool want = need && LB < best;
-                if (LB == NDBX_NO_CLUSTER || __ballot(want) == 0) continue;
-                float qw[D];
+            for (int k = 0; k < D; ++k) {
+                qx[k] = soa[k * npad + q];
+                lo[k] = wave_min(need ? qx[k] : __builtin_inff());
+                hi[k] = wave_max(need ? qx[k] : -__builtin_inff());
+            }
+            const int2 w = tile_window<D>(lane, rows_per_cell, tile.x, tile.y, lo, hi, g, cell_c[nc + tile.x],
+                                          cell_c[2 * nc + tile.x], ranges, cell_c, nc);
+            // the windows' cells flattened over the lanes (a scan of their
+            // lengths): each lane loads one cell's id and point range, so the
+            // serial walk below visits only cells that can lower some
+            // searching point's id, with no load before its tests
+            const int len = w.y > w.x ? w.y - w.x : 0;
+            int incl = len;
 #pragma unroll
-                for (int k = 0; k < D; ++k) qw[k] = want ? qx[k] : __builtin_nanf("");
-                const int b1 = cell_start[B + 1];
-                bool got = false;
-                int j = cell_start[B];
-                bool open = true;  // some wanting lane has no hit yet (wave-uniform)
-                for (; j + 8 <= b1 && open; j += 8) {
-                    got |= any_block<D, 8>(qw, core_pts, j, e2);
-                    open = __ballot(want && !got) != 0;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t2 = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += t2;
+            }
+            const int total = __builtin_amdgcn_readlane(incl, 63);
+            int32_t best_max = need ? best : INT_MIN;
+            for (int o = 32; o >= 1; o >>= 1) best_max = max(best_max, __shfl_xor(best_max, o, 64));
+            for (int base = 0; base < total; base += 64) {
+                const int k = base + lane;
+                // the window holding flattened cell k: the lane r with incl[r] > k, lowest such
+                int lo_w = 0, hi_w = 63;
+                while (lo_w < hi_w) {
+                    const int mid = (lo_w + hi_w) >> 1;
+                    if (__shfl(incl, mid, 64) > k) hi_w = mid;
+                    else lo_w = mid + 1;
                 }
-                for (; j < b1 && open; ++j) {
-                    got |= any_block<D, 1>(qw, core_pts, j, e2);
-                    open = __ballot(want && !got) != 0;
+                const int wx = __shfl(w.x, lo_w, 64), wi = __shfl(incl, lo_w, 64), wl = __shfl(len, lo_w, 64);
+                const int B = k < total ? wx + k - (wi - wl) : -1;
+                const int32_t LB = B >= 0 ? cell_label[B] : NDBX_NO_CLUSTER;
+                const bool useful = LB != NDBX_NO_CLUSTER && LB < best_max;
+                const int j0 = useful ? cell_start[B] : 0, j1 = useful ? cell_start[B + 1] : 0;
+                uint64_t um = __ballot(useful);
+                while (um) {
+                    const int src = __builtin_ctzll(um);
+                    um &= um - 1;
+                    const int32_t LBs = __builtin_amdgcn_readlane(LB, src);
+                    // only lanes that would improve take part; others see NaN
+                    const bool want = need && LBs < best;
+                    if (__ballot(want) == 0) continue;
+                    float qw[D];
+#pragma unroll
+                    for (int kk = 0; kk < D; ++kk) qw[kk] = want ? qx[kk] : __builtin_nanf("");
+                    const int b1 = __builtin_amdgcn_readlane(j1, src);
+                    bool got = false;
+                    int j = __builtin_amdgcn_readlane(j0, src);
+                    bool open = true;  // some wanting lane has no hit yet (wave-uniform)
+                    for (; j + 8 <= b1 && open; j += 8) {
+                        got |= any_block<D, 8>(qw, core_pts, j, e2);
+                        open = __ballot(want && !got) != 0;
+                    }
+                    for (; j < b1 && open; ++j) {
+                        got |= any_block<D, 1>(qw, core_pts, j, e2);
+                        open = __ballot(want && !got) != 0;
+                    }
+                    if (got) best = LBs;
                 }
-                if (got) best = LB;
             }
         }
+        if (need && best != own) labels[perm[q]] = best;
     }
-    if (valid) labels[perm[q]] = best;
 }
 
 // Per-instance (cores, noise, clusters) from input-order flags and labels;
@@ -681,7 +839,7 @@ static void grid_label_d(Ctx &c) {
     c.ctile_off.ensure(nc * 4);
     int32_t *big = c.gflag.as<int32_t>(), *small = c.ctile_off.as<int32_t>();
     int32_t *counts = c.misc.as<int32_t>() + 8;
-    fill_i32(counts, 0, 2, s, c.prof);
+    fill_i32(counts, 0, 4, s, c.prof);  // big list, small list, border tiles, border points
     {
         Launch L(c.prof, s, "cell_lists");
         k_cell_lists<<<ncb, 256, 0, s>>>(c.cell_has_core.as<int32_t>(), c.cell_start.as<int32_t>(), ncp, big, small,
@@ -715,11 +873,23 @@ static void grid_label_d(Ctx &c) {
     {
         Launch L(c.prof, s, "border");
         const int64_t nct = c.capacity_mode ? c.cap_tiles : c.n_tiles;
-        k_border<D><<<(unsigned)ceil_div(nct, 4), 256, 0, s>>>(
-            c.ctiles.as<int4>(), ntp, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig, c.lin_bits,
-            c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), ncp, c.cell_of.as<int32_t>(), c.soa.as<float>(),
-            c.soa_core.as<float>(), npad, c.core_s.as<uint8_t>(), c.nn_s.as<int32_t>(), c.cell_label.as<int32_t>(),
-            c.vals.as<uint32_t>(), c.labels.as<int32_t>());
+        c.border_tiles.ensure((size_t)(std::max<int64_t>(nct, 1) + n) * 4);
+        int32_t *blist = c.border_tiles.as<int32_t>(), *n_blist = counts + 2;
+        int32_t *plist = blist + std::max<int64_t>(nct, 1), *n_plist = counts + 3;
+        k_border_own<<<nb, 256, 0, s>>>(c.cell_of.as<int32_t>(), c.cell_label.as<int32_t>(), c.vals.as<uint32_t>(),
+                                        n, c.labels.as<int32_t>());
+        k_border_list<<<(unsigned)ceil_div(std::max<int64_t>(nct, 1), 256), 256, 0, s>>>(
+            c.ctiles.as<int4>(), ntp, c.core_s.as<uint8_t>(), c.nn_s.as<int32_t>(), blist, n_blist, plist, n_plist);
+        k_border_pts<D><<<nb, 256, 0, s>>>(plist, n_plist, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig,
+                                           c.lin_bits, c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), ncp,
+                                           c.cell_of.as<int32_t>(), c.soa.as<float>(), c.soa_core.as<float>(), npad,
+                                           c.cell_label.as<int32_t>(), c.vals.as<uint32_t>(), c.labels.as<int32_t>());
+        // waves loop over the list (its length is on the device)
+        k_border<D><<<(unsigned)std::min<int64_t>(ceil_div(std::max<int64_t>(nct, 1), 4), 4096), 256, 0, s>>>(
+            c.ctiles.as<int4>(), blist, n_blist, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig,
+            c.lin_bits, c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), ncp, c.cell_of.as<int32_t>(),
+            c.soa.as<float>(), c.soa_core.as<float>(), npad, c.core_s.as<uint8_t>(), c.nn_s.as<int32_t>(),
+            c.cell_label.as<int32_t>(), c.vals.as<uint32_t>(), c.labels.as<int32_t>());
     }
     fill_i32(c.meta.as<int32_t>(), 0, 3 * c.n_inst, s, c.prof);
     {
