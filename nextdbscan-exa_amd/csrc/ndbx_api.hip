@@ -21,7 +21,7 @@ Ctx::~Ctx() {
     if (graph) (void)hipGraphDestroy(graph);
     prof.destroy();
     DevBuf *bufs[] = {&own_coords, &nn, &labels, &core, &meta, &offsets_d, &inst_d, &bounds_part, &keys, &vals,
-                      &keys2, &vals2, &flag, &cid, &cell_of, &cell_start, &cell_key, &soa, &row_keys, &row_vals,
+                      &keys2, &vals2, &flag, &cid, &cell_of, &cell_start, &cell_key, &soa, &row_keys,
                       &nranges, &ranges, &tile_off, &tiles, &nn_s, &core_s, &cell_core_min, &cell_has_core,
                       &parent, &comp_min, &cell_label, &label_s, &rank, &misc, &scratch, &scratch2, &b_parent,
                       &b_rootmin, &soa_core, &cell_c, &gflag, &ctile_off, &ctiles, &tile_tests, &dense_id, &dense_info, &dense_tmp, &fine_off, &dcount, &plan_in, &part_off,

@@ -399,10 +399,16 @@ __device__ __forceinline__ uint64_t row_key_of(uint64_t key, int lin_bits, const
     return key - lin % (uint64_t)ig[inst].G[0];
 }
 
+// Row table slot: the row key and the row's cell range [first, end), 16
+// bytes, so a lookup that finds its row reads one line.
+struct RowSlot {
+    uint64_t key;
+    int32_t first, end;
+};
+
 __global__ __launch_bounds__(256) void k_row_insert(const uint64_t *__restrict__ cell_key, const int32_t *__restrict__ ncp,
                                                     const InstGrid *__restrict__ ig, int lin_bits,
-                                                    uint64_t *__restrict__ tkeys, int2 *__restrict__ tvals,
-                                                    uint64_t mask) {
+                                                    RowSlot *__restrict__ table, uint64_t mask) {
     const int64_t nc = *ncp;
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
@@ -412,22 +418,22 @@ __global__ __launch_bounds__(256) void k_row_insert(const uint64_t *__restrict__
     if (!first && !last) return;
     uint64_t slot = mix64(rk) & mask;
     for (;;) {
-        unsigned long long prev = atomicCAS((unsigned long long *)&tkeys[slot], (unsigned long long)kEmpty,
+        unsigned long long prev = atomicCAS((unsigned long long *)&table[slot].key, (unsigned long long)kEmpty,
                                             (unsigned long long)rk);
         if (prev == kEmpty || prev == rk) break;
         slot = (slot + 1) & mask;
     }
-    if (first) tvals[slot].x = (int32_t)c;
-    if (last) tvals[slot].y = (int32_t)(c + 1);
+    if (first) table[slot].first = (int32_t)c;
+    if (last) table[slot].end = (int32_t)(c + 1);
 }
 
-__device__ __forceinline__ bool row_lookup(const uint64_t *__restrict__ tkeys, const int2 *__restrict__ tvals,
-                                           uint64_t mask, uint64_t rk, int2 *out) {
+__device__ __forceinline__ bool row_lookup(const RowSlot *__restrict__ table, uint64_t mask, uint64_t rk, int2 *out) {
     uint64_t slot = mix64(rk) & mask;
     for (;;) {
-        const uint64_t k = tkeys[slot];
+        const uint4 e = reinterpret_cast<const uint4 *>(table)[slot];  // one 16-byte load
+        const uint64_t k = (uint64_t)e.y << 32 | e.x;
         if (k == rk) {
-            *out = tvals[slot];
+            *out = make_int2((int32_t)e.z, (int32_t)e.w);
             return true;
         }
         if (k == kEmpty) return false;
@@ -446,8 +452,8 @@ template <int D>
 __global__ __launch_bounds__(256) void k_stencil(const uint64_t *__restrict__ cell_key,
                                                  const int32_t *__restrict__ cell_start, const int32_t *__restrict__ ncp,
                                                  const InstGrid *__restrict__ ig, int lin_bits,
-                                                 const uint64_t *__restrict__ tkeys, const int2 *__restrict__ tvals,
-                                                 uint64_t mask, int2 *__restrict__ ranges, int rows_per_cell,
+                                                 const RowSlot *__restrict__ table, uint64_t mask,
+                                                 int2 *__restrict__ ranges, int rows_per_cell,
                                                  int32_t *__restrict__ cell_c) {
     const int64_t nc = *ncp;
     // one thread per (cell, stencil row)
@@ -479,7 +485,7 @@ __global__ __launch_bounds__(256) void k_stencil(const uint64_t *__restrict__ ce
     int2 rv;
     if (z >= 0 && z < G2 && y >= 0 && y < G1) {
         const uint64_t rk = (inst << lin_bits) | (uint64_t)((z * G1 + y) * G0);
-        if (row_lookup(tkeys, tvals, mask, rk, &rv)) {
+        if (row_lookup(table, mask, rk, &rv)) {
             int a = rv.x, b = rv.y;  // first cell with key >= rk+lo0
             const uint64_t klo = rk + (uint64_t)lo0, khi = rk + (uint64_t)hi0;
             while (a < b) {
@@ -1382,13 +1388,12 @@ static void grid_classify_d(Ctx &c) {
     // row hash: power of two >= 2 * cells
     uint64_t cap = 1024;
     while (cap < (uint64_t)(2 * nc)) cap <<= 1;
-    c.row_keys.ensure(cap * 8);
-    c.row_vals.ensure(cap * 8);
-    fill_u64(c.row_keys.as<uint64_t>(), kEmpty, (int64_t)cap, s, c.prof);
+    c.row_keys.ensure(cap * sizeof(RowSlot));
+    fill_u64(c.row_keys.as<uint64_t>(), kEmpty, (int64_t)cap * 2, s, c.prof);  // keys (and ranges) all ones
     {
         Launch L(c.prof, s, "row_insert");
-        k_row_insert<<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), ncp, ig, c.lin_bits, c.row_keys.as<uint64_t>(),
-                                         c.row_vals.as<int2>(), cap - 1);
+        k_row_insert<<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), ncp, ig, c.lin_bits, c.row_keys.as<RowSlot>(),
+                                         cap - 1);
     }
     const int rows = c.rows_per_cell = D == 3 ? 25 : D == 2 ? 5 : 1;
     c.ranges.ensure((size_t)nc * rows * 8);
@@ -1396,7 +1401,7 @@ static void grid_classify_d(Ctx &c) {
     {
         Launch L(c.prof, s, "stencil");
         k_stencil<D><<<(unsigned)ceil_div(nc * rows, 256), 256, 0, s>>>(c.cell_key.as<uint64_t>(), c.cell_start.as<int32_t>(), ncp, ig, c.lin_bits,
-                                         c.row_keys.as<uint64_t>(), c.row_vals.as<int2>(), cap - 1,
+                                         c.row_keys.as<RowSlot>(), cap - 1,
                                          c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>());
     }
     // count tiles over groups of cells of one row
