@@ -393,10 +393,39 @@ __device__ __forceinline__ uint64_t mix64(uint64_t v) {
 
 constexpr uint64_t kEmpty = ~0ull;
 
-__device__ __forceinline__ uint64_t row_key_of(uint64_t key, int lin_bits, const InstGrid *__restrict__ ig) {
-    const uint64_t inst = key >> lin_bits;
+// Stencil rows per cell, (2 reach + 1)^(D-1) at reach 2.
+template <int D>
+constexpr int kStencilRows = D == 3 ? 25 : D == 2 ? 5 : 1;
+
+// Cell coordinates (c0, c1, c2) of every cell from its key, as three planes
+// of cell_c, decoded once per cell (32-bit division when the key's linear
+// index and the grid sides fit, as they do unless an instance has 2^32 cells).
+__global__ __launch_bounds__(256) void k_cell_coords(const uint64_t *__restrict__ cell_key, const int32_t *__restrict__ ncp,
+                                                     const InstGrid *__restrict__ ig, int lin_bits,
+                                                     int32_t *__restrict__ cell_c) {
+    const int64_t nc = *ncp;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const uint64_t key = cell_key[c];
+    const InstGrid &g = ig[key >> lin_bits];
     const uint64_t lin = key & ((1ull << lin_bits) - 1);
-    return key - lin % (uint64_t)ig[inst].G[0];
+    const uint64_t G0 = (uint64_t)g.G[0], G1 = (uint64_t)g.G[1];
+    uint64_t c0, c1, c2;
+    if (((lin | G0 | G1) >> 32) == 0) {
+        const uint32_t l = (uint32_t)lin, g0 = (uint32_t)G0, g1 = (uint32_t)G1;
+        const uint32_t r = l / g0;
+        c0 = l - r * g0;
+        c1 = r % g1;
+        c2 = r / g1;
+    } else {
+        const uint64_t r = lin / G0;
+        c0 = lin - r * G0;
+        c1 = r % G1;
+        c2 = r / G1;
+    }
+    cell_c[c] = (int32_t)c0;
+    cell_c[nc + c] = (int32_t)c1;
+    cell_c[2 * nc + c] = (int32_t)c2;
 }
 
 // Row table slot: the row key and the row's cell range [first, end), 16
@@ -406,15 +435,16 @@ struct RowSlot {
     int32_t first, end;
 };
 
+// A cell's row key is its key minus its c0.
 __global__ __launch_bounds__(256) void k_row_insert(const uint64_t *__restrict__ cell_key, const int32_t *__restrict__ ncp,
-                                                    const InstGrid *__restrict__ ig, int lin_bits,
+                                                    const int32_t *__restrict__ cell_c,
                                                     RowSlot *__restrict__ table, uint64_t mask) {
     const int64_t nc = *ncp;
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
-    const uint64_t rk = row_key_of(cell_key[c], lin_bits, ig);
-    const bool first = (c == 0) || row_key_of(cell_key[c - 1], lin_bits, ig) != rk;
-    const bool last = (c == nc - 1) || row_key_of(cell_key[c + 1], lin_bits, ig) != rk;
+    const uint64_t rk = cell_key[c] - (uint64_t)cell_c[c];
+    const bool first = (c == 0) || cell_key[c - 1] - (uint64_t)cell_c[c - 1] != rk;
+    const bool last = (c == nc - 1) || cell_key[c + 1] - (uint64_t)cell_c[c + 1] != rk;
     if (!first && !last) return;
     uint64_t slot = mix64(rk) & mask;
     for (;;) {
@@ -453,29 +483,19 @@ __global__ __launch_bounds__(256) void k_stencil(const uint64_t *__restrict__ ce
                                                  const int32_t *__restrict__ cell_start, const int32_t *__restrict__ ncp,
                                                  const InstGrid *__restrict__ ig, int lin_bits,
                                                  const RowSlot *__restrict__ table, uint64_t mask,
-                                                 int2 *__restrict__ ranges, int rows_per_cell,
-                                                 int32_t *__restrict__ cell_c) {
+                                                 int2 *__restrict__ ranges, const int32_t *__restrict__ cell_c) {
+    constexpr int R = kStencilRows<D>, k = 2;
     const int64_t nc = *ncp;
     // one thread per (cell, stencil row)
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= nc * rows_per_cell) return;
-    const int64_t c = tid / rows_per_cell;
-    const int slot = (int)(tid - c * rows_per_cell);
+    if (tid >= nc * R) return;
+    const int64_t c = tid / R;
+    const int slot = (int)(tid - c * R);
     const uint64_t key = cell_key[c];
     const uint64_t inst = key >> lin_bits;
     const InstGrid &g = ig[inst];
-    uint64_t lin = key & ((1ull << lin_bits) - 1);
     const int64_t G0 = g.G[0], G1 = g.G[1], G2 = g.G[2];
-    const int64_t c0 = (int64_t)(lin % (uint64_t)G0);
-    lin /= (uint64_t)G0;
-    const int64_t c1 = D >= 2 ? (int64_t)(lin % (uint64_t)G1) : 0;
-    const int64_t c2 = D >= 3 ? (int64_t)(lin / (uint64_t)G1) : 0;
-    if (slot == 0) {
-        cell_c[c] = (int32_t)c0;
-        cell_c[nc + c] = (int32_t)c1;
-        cell_c[2 * nc + c] = (int32_t)c2;
-    }
-    const int k = g.reach;
+    const int64_t c0 = cell_c[c], c1 = cell_c[nc + c], c2 = cell_c[2 * nc + c];
     const int dz = D >= 3 ? slot / (2 * k + 1) - k : 0;
     const int dy = D >= 3 ? slot % (2 * k + 1) - k : (D == 2 ? slot - k : 0);
     const int64_t lo0 = c0 - k < 0 ? 0 : c0 - k;
@@ -502,7 +522,7 @@ __global__ __launch_bounds__(256) void k_stencil(const uint64_t *__restrict__ ce
             out = make_int2(a, e);
         }
     }
-    ranges[c * rows_per_cell + slot] = out;
+    ranges[c * R + slot] = out;
 }
 
 // Counting groups: the cells of one row whose c0 share c0 / group_cells; the
@@ -1388,21 +1408,25 @@ static void grid_classify_d(Ctx &c) {
     // row hash: power of two >= 2 * cells
     uint64_t cap = 1024;
     while (cap < (uint64_t)(2 * nc)) cap <<= 1;
+    c.cell_c.ensure((size_t)nc * 3 * 4);
+    {
+        Launch L(c.prof, s, "cell_coords");
+        k_cell_coords<<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), ncp, ig, c.lin_bits, c.cell_c.as<int32_t>());
+    }
     c.row_keys.ensure(cap * sizeof(RowSlot));
     fill_u64(c.row_keys.as<uint64_t>(), kEmpty, (int64_t)cap * 2, s, c.prof);  // keys (and ranges) all ones
     {
         Launch L(c.prof, s, "row_insert");
-        k_row_insert<<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), ncp, ig, c.lin_bits, c.row_keys.as<RowSlot>(),
-                                         cap - 1);
+        k_row_insert<<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), ncp, c.cell_c.as<int32_t>(),
+                                         c.row_keys.as<RowSlot>(), cap - 1);
     }
-    const int rows = c.rows_per_cell = D == 3 ? 25 : D == 2 ? 5 : 1;
+    const int rows = c.rows_per_cell = kStencilRows<D>;
     c.ranges.ensure((size_t)nc * rows * 8);
-    c.cell_c.ensure((size_t)nc * 3 * 4);
     {
         Launch L(c.prof, s, "stencil");
-        k_stencil<D><<<(unsigned)ceil_div(nc * rows, 256), 256, 0, s>>>(c.cell_key.as<uint64_t>(), c.cell_start.as<int32_t>(), ncp, ig, c.lin_bits,
-                                         c.row_keys.as<RowSlot>(), cap - 1,
-                                         c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>());
+        k_stencil<D><<<(unsigned)ceil_div(nc * rows, 256), 256, 0, s>>>(
+            c.cell_key.as<uint64_t>(), c.cell_start.as<int32_t>(), ncp, ig, c.lin_bits, c.row_keys.as<RowSlot>(),
+            cap - 1, c.ranges.as<int2>(), c.cell_c.as<int32_t>());
     }
     // count tiles over groups of cells of one row
     c.gflag.ensure(nc * 4);
