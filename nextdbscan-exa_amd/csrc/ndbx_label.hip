@@ -195,8 +195,12 @@ __global__ __launch_bounds__(256) void k_union(const int32_t *__restrict__ list,
                                                const uint64_t *__restrict__ core_bits,
                                                int32_t *__restrict__ parent) {
     const int64_t nc = *ncp;
+    // per wave: candidate cells, their roots, their offsets from A (packed
+    // (dc0 + 2) + 5 (dc1 + 2) + 25 (dc2 + 2)) and point ranges
     __shared__ int32_t cand[4][kMaxStencilCells];
     __shared__ int32_t cand_root[4][kMaxStencilCells];
+    __shared__ int32_t cand_off[4][kMaxStencilCells];
+    __shared__ int2 cand_pts[4][kMaxStencilCells];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t li = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
     if (li >= *n_list) return;
@@ -205,9 +209,11 @@ __global__ __launch_bounds__(256) void k_union(const int32_t *__restrict__ list,
     const InstGrid &g = ig[cell_key[A] >> lin_bits];
     const float e2 = g.e2;
     const int a0 = cell_start[A], a1 = cell_start[A + 1];
-    const int ac0 = cell_c[A], ac1 = cell_c[nc + A], ac2 = cell_c[2 * nc + A];
+    const int ac[3] = {cell_c[A], cell_c[nc + A], cell_c[2 * nc + A]};
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const int center = rows_per_cell / 2;  // slot of (dz, dy) = (0, 0)
+    // the slot's row offsets (the stencil reaches 2 cells)
+    const int dy = D >= 2 ? (D == 3 ? lane % 5 : lane) - 2 : 0, dz = D == 3 ? lane / 5 - 2 : 0;
 
     // ---- candidate list
     int n_cand = 0;
@@ -217,58 +223,76 @@ __global__ __launch_bounds__(256) void k_union(const int32_t *__restrict__ list,
         if (lane < D) {
             const int slot = lane == 0 ? center : (lane == 1 ? center + 1 : center + 5);
             const int2 w = ranges[(int64_t)A * rows_per_cell + slot];
-            const int want0 = lane == 0 ? ac0 + 1 : ac0;
+            const int want0 = lane == 0 ? ac[0] + 1 : ac[0];
             for (int b = w.x; b < w.y; ++b)
                 if (cell_c[b] == want0) B = b;
             if (B >= 0 && !core_cell(core_bits, B)) B = -1;
         }
         const uint64_t mask = __ballot(B >= 0);
-        if (B >= 0) cand[wave][__popcll(mask & lt)] = B;
+        if (B >= 0) {
+            const int i = __popcll(mask & lt);
+            cand[wave][i] = B;
+            cand_off[wave][i] = 62 + (lane == 0 ? 1 : lane == 1 ? 5 : 25);  // (2, 2, 2) + e_lane
+            cand_pts[wave][i] = make_int2(cell_start[B], cell_start[B + 1]);
+        }
         n_cand = __popcll(mask);
     } else {
+        // rows before the centre hold only cells below A
         int2 my = make_int2(0, 0);
-        if (lane < rows_per_cell) my = ranges[(int64_t)A * rows_per_cell + lane];
-        // gather B > A with a core, grouped nearest first by the squared gap
-        // in cells between A and B: sum over k of max(|dc_k| - 1, 0)^2
-        int Bs[5], cls[5];
+        if (lane >= center && lane < rows_per_cell) my = ranges[(int64_t)A * rows_per_cell + lane];
+        // gather B > A with a core, grouped nearest first by the number of
+        // dimensions in which B is more than one cell from A
+        int Bs[5], cls[5], offs[5];
+        const int cl12 = (abs(dy) > 1) + (abs(dz) > 1);
 #pragma unroll
         for (int it = 0; it < 5; ++it) {
             const int B = my.x + it;
             const bool keep = B < my.y && B > A && core_cell(core_bits, B);
             Bs[it] = keep ? B : -1;
-            int cl = 0;
+            int cl = 0, off = 0;
             if (keep) {
-                const int d0 = abs(cell_c[B] - ac0), d1 = abs(cell_c[nc + B] - ac1), d2 = abs(cell_c[2 * nc + B] - ac2);
-                cl = (d0 > 1) + (d1 > 1) + (d2 > 1);
+                const int d0 = cell_c[B] - ac[0];
+                cl = (abs(d0) > 1) + cl12;
+                off = (d0 + 2) + 5 * (dy + 2) + 25 * (dz + 2);
             }
             cls[it] = cl;
+            offs[it] = off;
         }
         for (int want = 0; want <= D; ++want) {
 #pragma unroll
             for (int it = 0; it < 5; ++it) {
                 const bool take = Bs[it] >= 0 && cls[it] == want;
                 const uint64_t mask = __ballot(take);
-                if (take) cand[wave][n_cand + __popcll(mask & lt)] = Bs[it];
+                if (take) {
+                    cand[wave][n_cand + __popcll(mask & lt)] = Bs[it];
+                    cand_off[wave][n_cand + __popcll(mask & lt)] = offs[it];
+                }
                 n_cand += __popcll(mask);
             }
         }
         __builtin_amdgcn_wave_barrier();
-        // drop candidates already in A's set (one lane per candidate)
+        // drop candidates already in A's set (one lane per candidate); the
+        // kept ones take their point ranges along
         const int32_t root_a = uf_find(parent, A);
         int kept = 0;
         for (int base = 0; base < n_cand; base += 64) {
             const int ci = base + lane;
-            int B = -1, rb = -1;
+            int B = -1, rb = -1, off = 0;
             if (ci < n_cand) {
                 B = cand[wave][ci];
+                off = cand_off[wave][ci];
                 rb = uf_find(parent, B);
                 if (rb == root_a) B = -1;
             }
+            const int2 pts = B >= 0 ? make_int2(cell_start[B], cell_start[B + 1]) : make_int2(0, 0);
             __builtin_amdgcn_wave_barrier();
             const uint64_t mask = __ballot(B >= 0);
             if (B >= 0) {
-                cand[wave][kept + __popcll(mask & lt)] = B;
-                cand_root[wave][kept + __popcll(mask & lt)] = rb;
+                const int i = kept + __popcll(mask & lt);
+                cand[wave][i] = B;
+                cand_root[wave][i] = rb;
+                cand_off[wave][i] = off;
+                cand_pts[wave][i] = pts;
             }
             kept += __popcll(mask);
         }
@@ -304,15 +328,19 @@ __global__ __launch_bounds__(256) void k_union(const int32_t *__restrict__ list,
                 if (seen) continue;
             }
             // box of this chunk's cores against B's cell box
+            const int off = __builtin_amdgcn_readfirstlane(cand_off[wave][ci]);
+            const int bc[3] = {ac[0] + off % 5 - 2, ac[1] + off / 5 % 5 - 2, ac[2] + off / 25 - 2};
             double gap2 = 0.0;
 #pragma unroll
             for (int k = 0; k < D; ++k) {
-                const double blo = g.origin[k] + (double)cell_c[k * nc + B] * s - slack, bhi = blo + s + 2.0 * slack;
+                const double blo = g.origin[k] + (double)bc[k] * s - slack, bhi = blo + s + 2.0 * slack;
                 const double gk = fmax(0.0, fmax(blo - (double)hi[k], (double)lo[k] - bhi));
                 gap2 += gk * gk;
             }
             if (__builtin_amdgcn_readfirstlane(gap2 > r2)) continue;
-            if (wave_any_within<D>(qx, core_pts, cell_start[B], cell_start[B + 1], e2)) {
+            const int b0 = __builtin_amdgcn_readfirstlane(cand_pts[wave][ci].x);
+            const int b1 = __builtin_amdgcn_readfirstlane(cand_pts[wave][ci].y);
+            if (wave_any_within<D>(qx, core_pts, b0, b1, e2)) {
                 if (lane == 0) {
                     cand[wave][ci] = -1;
                     uf_unite(parent, A, B);
