@@ -99,6 +99,42 @@ __device__ __forceinline__ bool core_cell(const uint64_t *__restrict__ core_bits
     return (core_bits[c >> 6] >> (c & 63)) & 1;
 }
 
+// Entries of a global list reserved for the whole block with one atomic
+// (atomics on one counter serialise, so list kernels take one per block, not
+// per wave): `cnt` is the calling wave's count (wave-uniform); returns the
+// wave's first entry.  Every thread of the block calls it; `sh` is NW + 1
+// ints of LDS.
+template <int NW>
+__device__ __forceinline__ int32_t block_reserve(int32_t cnt, int32_t *counter, int32_t *sh) {
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[wave] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t tot = 0;
+        for (int w = 0; w < NW; ++w) {
+            const int32_t t = sh[w];
+            sh[w] = tot;
+            tot += t;
+        }
+        sh[NW] = tot ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
+    const int32_t r = sh[NW] + sh[wave];
+    __syncthreads();
+    return r;
+}
+
+// Every listed cell hung directly under its root (between the union
+// launches: the second one's finds then take one step).
+__global__ __launch_bounds__(256) void k_compress(const int32_t *__restrict__ list, const int32_t *__restrict__ n_list,
+                                                  int32_t *__restrict__ parent) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *n_list) return;
+    const int32_t c = list[i];
+    const int32_t r = uf_find(parent, c);
+    if (r != c) atomicMin(parent + c, r);  // parents only decrease
+}
+
 __global__ __launch_bounds__(256) void k_iota(int32_t *__restrict__ p, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = (int32_t)i;
@@ -356,37 +392,40 @@ __global__ __launch_bounds__(256) void k_union(const int32_t *__restrict__ list,
 // wave-per-cell union, the rest to the lane-per-cell union.
 constexpr int kSmallCell = 8;
 
+constexpr int kListPer = 8;  // cells per thread of the list kernel
+
 __global__ __launch_bounds__(256) void k_cell_lists(const int32_t *__restrict__ cell_has_core,
                                                     const int32_t *__restrict__ cell_start, const int32_t *__restrict__ ncp,
                                                     int32_t *__restrict__ big, int32_t *__restrict__ small,
                                                     int32_t *__restrict__ counts) {
     const int64_t nc = *ncp;
-    __shared__ int32_t wb[4], ws[4], bb, bs;
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool v = c < nc && cell_has_core[c];
-    const bool is_big = v && cell_start[c + 1] - cell_start[c] > kSmallCell;
-    const bool is_small = v && !is_big;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __shared__ int32_t shb[5], shs[5];
+    const int lane = threadIdx.x & 63;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const uint64_t mb = __ballot(is_big), ms = __ballot(is_small);
-    if (lane == 0) {
-        wb[w] = __popcll(mb);
-        ws[w] = __popcll(ms);
+    // wave w takes 64 kListPer consecutive cells, so the lists stay in cell order
+    const int64_t c0 = (int64_t)blockIdx.x * (256 * kListPer) + (threadIdx.x >> 6) * (64 * kListPer) + lane;
+    uint64_t mb[kListPer], ms[kListPer];
+    int32_t nb = 0, ns = 0;
+#pragma unroll
+    for (int it = 0; it < kListPer; ++it) {
+        const int64_t c = c0 + it * 64;
+        const bool v = c < nc && cell_has_core[c];
+        const bool is_big = v && cell_start[c + 1] - cell_start[c] > kSmallCell;
+        mb[it] = __ballot(is_big);
+        ms[it] = __ballot(v && !is_big);
+        nb += __popcll(mb[it]);
+        ns += __popcll(ms[it]);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {  // one atomic per block and list
-        const int tb = wb[0] + wb[1] + wb[2] + wb[3], ts = ws[0] + ws[1] + ws[2] + ws[3];
-        bb = tb ? atomicAdd(counts + 0, tb) : 0;
-        bs = ts ? atomicAdd(counts + 1, ts) : 0;
+    int32_t ob = block_reserve<4>(nb, counts + 0, shb);
+    int32_t os = block_reserve<4>(ns, counts + 1, shs);
+#pragma unroll
+    for (int it = 0; it < kListPer; ++it) {
+        const int32_t c = (int32_t)(c0 + it * 64);
+        if ((mb[it] >> lane) & 1) big[ob + __popcll(mb[it] & lt)] = c;
+        if ((ms[it] >> lane) & 1) small[os + __popcll(ms[it] & lt)] = c;
+        ob += __popcll(mb[it]);
+        os += __popcll(ms[it]);
     }
-    __syncthreads();
-    int ob = bb, os = bs;
-    for (int i = 0; i < w; ++i) {
-        ob += wb[i];
-        os += ws[i];
-    }
-    if (is_big) big[ob + __popcll(mb & lt)] = (int32_t)c;
-    if (is_small) small[os + __popcll(ms & lt)] = (int32_t)c;
 }
 
 // One lane per small core cell A: every stencil neighbour B > A with a core
@@ -533,46 +572,67 @@ __global__ __launch_bounds__(256) void k_border_own(const int32_t *__restrict__ 
     if (i < n) labels[perm[i]] = cell_label[cell_of[i]];  // NO_CLUSTER when the cell has no core
 }
 
-__global__ __launch_bounds__(256) void k_border_list(const int4 *__restrict__ ctiles, const int32_t *__restrict__ ntp,
+constexpr int kBorderListThreads = 1024;
+
+__global__ __launch_bounds__(kBorderListThreads) void k_border_list(const int4 *__restrict__ ctiles, const int32_t *__restrict__ ntp,
                                                      const uint8_t *__restrict__ core_s,
                                                      const int32_t *__restrict__ nn_s, int32_t *__restrict__ list,
                                                      int32_t *__restrict__ n_list, int32_t *__restrict__ plist,
                                                      int32_t *__restrict__ n_plist) {
+    constexpr int NW = kBorderListThreads / 64;
+    __shared__ int32_t sh_t[NW + 1], sh_p[NW + 1];
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool wave_tile = false;
-    int np = 0;  // searching points of a small tile
-    int4 tile = make_int4(0, 0, 0, 0);
-    if (t < *ntp) {
-        tile = ctiles[t];
-        if (tile.w > kBorderLaneTile) {
-            for (int q = tile.z; q < tile.z + tile.w && !wave_tile; ++q) wave_tile = !core_s[q] && nn_s[q] > 1;
-        } else {
-            for (int q = tile.z; q < tile.z + tile.w; ++q) np += !core_s[q] && nn_s[q] > 1;
-        }
-    }
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const int4 tile = t < *ntp ? ctiles[t] : make_int4(0, 0, 0, 0);
+    // a large tile: its lane walks the points until one searches
+    bool wave_tile = false;
+    if (tile.w > kBorderLaneTile)
+        for (int q = tile.z; q < tile.z + tile.w && !wave_tile; ++q) wave_tile = !core_s[q] && nn_s[q] > 1;
     const uint64_t m = __ballot(wave_tile);
-    if (m != 0) {
-        int32_t base = 0;
-        if (lane == 0) base = atomicAdd(n_list, __popcll(m));
-        base = __shfl(base, 0, 64);
-        if (wave_tile) list[base + __popcll(m & lt)] = (int32_t)t;
-    }
-    if (__ballot(np != 0) != 0) {
-        int incl = np;
+    const int32_t tbase = block_reserve<NW>(__popcll(m), n_list, sh_t);
+    if (wave_tile) list[tbase + __popcll(m & lt)] = (int32_t)t;
+    // the small tiles' points flattened over the wave (consecutive tiles hold
+    // consecutive points, so the loads coalesce), one lane each; the wave's
+    // searching points are reserved with one atomic, so the list keeps
+    // neighbouring points together
+    const int len = tile.w <= kBorderLaneTile ? tile.w : 0;
+    int incl = len;
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int t2 = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += t2;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t2 = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t2;
+    }
+    const int total = __builtin_amdgcn_readlane(incl, 63);  // <= 64 kBorderLaneTile
+    constexpr int kSteps = kBorderLaneTile;
+    uint64_t pm[kSteps];
+    int qs[kSteps];
+    int n_need = 0;
+#pragma unroll
+    for (int it = 0; it < kSteps; ++it) {
+        pm[it] = 0;
+        qs[it] = 0;
+        const int k = it * 64 + lane;
+        if (it * 64 >= total) continue;  // wave-uniform
+        int lo_w = 0, hi_w = 63;  // lowest lane with incl > k
+#pragma unroll
+        for (int st = 0; st < 6; ++st) {
+            const int mid = (lo_w + hi_w) >> 1;
+            if (__shfl(incl, mid, 64) > k) hi_w = mid;
+            else lo_w = mid + 1;
         }
-        int32_t base = 0;
-        if (lane == 63) base = atomicAdd(n_plist, incl);
-        base = __shfl(base, 63, 64);
-        int pos = base + incl - np;
-        if (np)
-            for (int q = tile.z; q < tile.z + tile.w; ++q)
-                if (!core_s[q] && nn_s[q] > 1) plist[pos++] = q;
+        const int wz = __shfl(tile.z, lo_w, 64), wi = __shfl(incl, lo_w, 64), wl = __shfl(len, lo_w, 64);
+        const int q = wz + k - (wi - wl);
+        const bool need = k < total && !core_s[q] && nn_s[q] > 1;
+        pm[it] = __ballot(need);
+        qs[it] = q;
+        n_need += __popcll(pm[it]);
+    }
+    int32_t base = block_reserve<NW>(n_need, n_plist, sh_p);
+#pragma unroll
+    for (int it = 0; it < kSteps; ++it) {
+        if ((pm[it] >> lane) & 1) plist[base + __popcll(pm[it] & lt)] = qs[it];
+        base += __popcll(pm[it]);
     }
 }
 
@@ -602,24 +662,10 @@ __global__ __launch_bounds__(256) void k_border_pts(const int32_t *__restrict__ 
     float qx[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) qx[k] = soa[k * npad + q];
-    const double inv_s = g.inv_s, s = 1.0 / inv_s, slack = s * 9.5367431640625e-07;  // 2^-20 cell
-    const double r2 = search_r2(e2);
     const int c1 = D >= 2 ? cell_c[nc + A] : 0, c2 = D >= 3 ? cell_c[2 * nc + A] : 0;
     for (int slot = 0; slot < rows_per_cell; ++slot) {
-        double rem = r2;
-        if (D >= 2) {
-            const int dy = (D == 3 ? slot % 5 : slot) - 2;
-            const double ylo = g.origin[1] + (double)(c1 + dy) * s - slack, yhi = ylo + s + 2.0 * slack;
-            const double gy = fmax(0.0, fmax(ylo - (double)qx[1], (double)qx[1] - yhi));
-            rem -= gy * gy;
-        }
-        if (D >= 3) {
-            const int dz = slot / 5 - 2;
-            const double zlo = g.origin[2] + (double)(c2 + dz) * s - slack, zhi = zlo + s + 2.0 * slack;
-            const double gz = fmax(0.0, fmax(zlo - (double)qx[2], (double)qx[2] - zhi));
-            rem -= gz * gz;
-        }
-        if (rem < 0.0) continue;
+        int xl, xh;
+        if (!row_reach<D>(slot, qx, qx, g, c1, c2, xl, xh)) continue;
         const int2 w = ranges[(int64_t)A * rows_per_cell + slot];
         for (int B = w.x; B < w.y; ++B) {
             const int32_t LB = cell_label[B];
@@ -870,10 +916,14 @@ static void grid_label_d(Ctx &c) {
     fill_i32(counts, 0, 4, s, c.prof);  // big list, small list, border tiles, border points
     {
         Launch L(c.prof, s, "cell_lists");
-        k_cell_lists<<<ncb, 256, 0, s>>>(c.cell_has_core.as<int32_t>(), c.cell_start.as<int32_t>(), ncp, big, small,
-                                         counts);
+        k_cell_lists<<<(unsigned)ceil_div(nc, 256 * kListPer), 256, 0, s>>>(
+            c.cell_has_core.as<int32_t>(), c.cell_start.as<int32_t>(), ncp, big, small, counts);
     }
     for (int phase = 0; phase < 2; ++phase) {
+        if (phase == 1) {
+            Launch L(c.prof, s, "compress");
+            k_compress<<<(unsigned)ceil_div(nc, 256), 256, 0, s>>>(big, counts, c.parent.as<int32_t>());
+        }
         Launch L(c.prof, s, phase == 0 ? "union_faces" : "union");
         k_union<D><<<(unsigned)ceil_div(nc, 4), 256, 0, s>>>(big, counts, ncp, phase, c.cell_start.as<int32_t>(),
                                                              c.cell_key.as<uint64_t>(), ig, c.lin_bits,
@@ -906,7 +956,7 @@ static void grid_label_d(Ctx &c) {
         int32_t *plist = blist + std::max<int64_t>(nct, 1), *n_plist = counts + 3;
         k_border_own<<<nb, 256, 0, s>>>(c.cell_of.as<int32_t>(), c.cell_label.as<int32_t>(), c.vals.as<uint32_t>(),
                                         n, c.labels.as<int32_t>());
-        k_border_list<<<(unsigned)ceil_div(std::max<int64_t>(nct, 1), 256), 256, 0, s>>>(
+        k_border_list<<<(unsigned)ceil_div(std::max<int64_t>(nct, 1), kBorderListThreads), kBorderListThreads, 0, s>>>(
             c.ctiles.as<int4>(), ntp, c.core_s.as<uint8_t>(), c.nn_s.as<int32_t>(), blist, n_blist, plist, n_plist);
         k_border_pts<D><<<nb, 256, 0, s>>>(plist, n_plist, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig,
                                            c.lin_bits, c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), ncp,

@@ -101,22 +101,14 @@ __device__ __forceinline__ uint32_t parts_in_reach(const float (&lo)[D], const f
     return mask;
 }
 
-// Candidate cells of one stencil row of a tile, computed by lane `slot`
-// (slot < rows_per_cell; other lanes return an empty window).  The tile is a
-// run of cells [cf, cl] of one row; its bounding box is [lo, hi].  Rows
-// whose perpendicular gap to the box exceeds the search radius are empty;
-// along dimension 0 only cells that can hold a point within the radius of the
-// box are kept.  Cell bounds are widened by 2^-20 of a cell, so no point the
-// predicate accepts is dropped.  Returns the cell index range [a, e).
+// Stencil row `slot` (reach 2) around a cell of row (c1, c2): whether a
+// point of the box [lo, hi] can have a point within the search radius in it,
+// and if so the cells along dimension 0 that can hold one, [xl, xh].  Cell
+// bounds are widened by 2^-20 of a cell, so no point the predicate accepts is
+// dropped.
 template <int D>
-__device__ __forceinline__ int2 tile_window(int slot, int rows_per_cell, int cf, int cl, const float (&lo)[D],
-                                            const float (&hi)[D], const InstGrid &g, int c1, int c2,
-                                            const int2 *__restrict__ ranges, const int32_t *__restrict__ cell_c,
-                                            int64_t nc) {
-    if (slot >= rows_per_cell) return make_int2(0, 0);
-    int a = ranges[(int64_t)cf * rows_per_cell + slot].x;
-    int e = ranges[(int64_t)cl * rows_per_cell + slot].y;
-    if (a >= e) return make_int2(0, 0);
+__device__ __forceinline__ bool row_reach(int slot, const float (&lo)[D], const float (&hi)[D], const InstGrid &g,
+                                          int c1, int c2, int &xl, int &xh) {
     const double inv_s = g.inv_s, s = 1.0 / inv_s, slack = s * 9.5367431640625e-07;  // 2^-20 cell
     double rem = search_r2(g.e2);
     if (D >= 2) {
@@ -131,10 +123,30 @@ __device__ __forceinline__ int2 tile_window(int slot, int rows_per_cell, int cf,
         const double gz = fmax(0.0, fmax(zlo - (double)hi[2], (double)lo[2] - zhi));
         rem -= gz * gz;
     }
-    if (rem < 0.0) return make_int2(0, 0);
+    if (rem < 0.0) return false;
     const double rx = sqrt(rem) + slack;
-    const int xl = (int)floor(((double)lo[0] - rx - g.origin[0]) * inv_s);
-    const int xh = (int)floor(((double)hi[0] + rx - g.origin[0]) * inv_s);
+    xl = (int)floor(((double)lo[0] - rx - g.origin[0]) * inv_s);
+    xh = (int)floor(((double)hi[0] + rx - g.origin[0]) * inv_s);
+    return true;
+}
+
+// Candidate cells of one stencil row of a tile, computed by lane `slot`
+// (slot < rows_per_cell; other lanes return an empty window).  The tile is a
+// run of cells [cf, cl] of one row; its bounding box is [lo, hi].  Rows out
+// of reach of the box are empty; along dimension 0 only cells that can hold
+// a point within the radius of the box are kept.  Returns the cell index
+// range [a, e).
+template <int D>
+__device__ __forceinline__ int2 tile_window(int slot, int rows_per_cell, int cf, int cl, const float (&lo)[D],
+                                            const float (&hi)[D], const InstGrid &g, int c1, int c2,
+                                            const int2 *__restrict__ ranges, const int32_t *__restrict__ cell_c,
+                                            int64_t nc) {
+    if (slot >= rows_per_cell) return make_int2(0, 0);
+    int a = ranges[(int64_t)cf * rows_per_cell + slot].x;
+    int e = ranges[(int64_t)cl * rows_per_cell + slot].y;
+    if (a >= e) return make_int2(0, 0);
+    int xl, xh;
+    if (!row_reach<D>(slot, lo, hi, g, c1, c2, xl, xh)) return make_int2(0, 0);
     while (a < e && cell_c[a] < xl) ++a;
     while (e > a && cell_c[e - 1] > xh) --e;
     return make_int2(a, e);
