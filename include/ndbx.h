@@ -126,9 +126,14 @@ int ndbx_profile_count(ndbx_ctx *ctx, int *count);
 int ndbx_profile_entry(ndbx_ctx *ctx, int i, const char **name, double *total_ms,
                        int64_t *launches);
 int ndbx_profile_reset(ndbx_ctx *ctx);
+/* Restricts the events to launches named `name` (NULL or "" = every
+ * launch), so a timed run can keep its one kernel of interest evented
+ * without the per-launch overhead of the others. */
+int ndbx_profile_select(ndbx_ctx *ctx, const char *name);
 
 /* Grid statistics of the last classify: non-empty cells, tiles, pair tests
- * issued by the counting kernel (0 when not collected). */
+ * issued by the counting kernel (collected while NDBX_PROFILE events every
+ * launch; otherwise the last collected value, 0 if none). */
 int ndbx_stats(ndbx_ctx *ctx, int64_t *cells, int64_t *tiles, int64_t *pair_tests);
 
 /* MFMA filter audit (NDBX_FP64_AUDIT): guard-band pairs rechecked, decisions

@@ -122,6 +122,7 @@ static void autotune(Ctx &c) {
     c.prof.ms.clear();
     c.prof.calls.clear();
     c.prof.on = true;
+    c.prof.only = "count";
     int best = c.group_cells;
     double best_ms = 1e300;
     for (int gc : {1, 2, 4, 8, 16}) {
@@ -457,6 +458,13 @@ int ndbx_profile_reset(ndbx_ctx *h) {
         need(h != nullptr, "ctx is null");
         h->c.sync();
         h->c.prof.reset();
+    });
+}
+
+int ndbx_profile_select(ndbx_ctx *h, const char *name) {
+    return guarded([&] {
+        need(h != nullptr, "ctx is null");
+        h->c.prof.only = name ? name : "";
     });
 }
 

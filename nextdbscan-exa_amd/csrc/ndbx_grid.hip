@@ -1459,7 +1459,7 @@ static void grid_classify_d(Ctx &c) {
             k_dense_tile_fill<<<(unsigned)c.n_dense, 256, 0, s>>>(dv, c.ctile_off.as<int32_t>(), c.ctiles.as<int4>());
     }
     c.nn_s.ensure(n * 4);
-    const bool stats = c.prof.on && !cap_mode;
+    const bool stats = c.prof.on && c.prof.only.empty() && !cap_mode;  // pair tests when every launch is evented
     if (stats) {
         c.tile_tests.ensure(nct * 8);
         fill_u64(reinterpret_cast<uint64_t *>(d_tests), 0, 1, s, c.prof);

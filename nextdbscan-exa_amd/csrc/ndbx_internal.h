@@ -49,6 +49,7 @@ struct DevBuf {
 // ------------------------------------------------------------- profiler --
 struct Profiler {
     bool on = false;
+    std::string only;  // non-empty: events only around launches of this name
     struct Pending {
         int slot;
         hipEvent_t a, b;

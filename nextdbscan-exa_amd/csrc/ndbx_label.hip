@@ -809,18 +809,30 @@ __global__ __launch_bounds__(256) void k_meta_points(const uint8_t *__restrict__
             b_acc += z;
             continue;
         }
-        int inst = 0;
-        if (v) {
+        // the instance of the wave's first point (a wave-uniform search), and
+        // of each lane only when the wave crosses into the next instance
+        int first = 0;
+        {
             int lo = 0, hi = n_inst - 1;
             while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
+                const int mid = (lo + hi + 1) >> 1;
+                if (off[mid] <= base) lo = mid;
+                else hi = mid - 1;
+            }
+            first = lo;
+        }
+        const int64_t last = base + 63 < n ? base + 63 : n - 1;
+        int inst = first;
+        if (off[first + 1] <= last && v) {
+            int lo = first, hi = n_inst - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
                 if (off[mid] <= i) lo = mid;
                 else hi = mid - 1;
             }
             inst = lo;
         }
-        const int first = __shfl(inst, 0, 64);
-        if (__ballot(v && inst != first) == 0) {
+        if (off[first + 1] > last) {
             int a = c, b = z;
             for (int o = 32; o >= 1; o >>= 1) {
                 a += __shfl_xor(a, o, 64);

@@ -73,7 +73,7 @@ void Profiler::destroy() {
 }
 
 Launch::Launch(Profiler &p, hipStream_t st, const char *name) : prof(p), s(st) {
-    if (!prof.on) return;
+    if (!prof.on || (!prof.only.empty() && prof.only != name)) return;
     slot = prof.slot_of(name);
     a = prof.take();
     b = prof.take();

@@ -53,6 +53,7 @@ SIGNATURES = {
     "ndbx_profile_count": ([_P, _IP], ctypes.c_int),
     "ndbx_profile_entry": ([_P, ctypes.c_int, _CPP, _DP, _I64P], ctypes.c_int),
     "ndbx_profile_reset": ([_P], ctypes.c_int),
+    "ndbx_profile_select": ([_P, ctypes.c_char_p], ctypes.c_int),
     "ndbx_stats": ([_P, _I64P, _I64P, _I64P], ctypes.c_int),
     "ndbx_audit": ([_P, _I64P, _I64P, _I64P, _DP], ctypes.c_int),
     "ndbx_destroy": ([_P], ctypes.c_int),
@@ -180,6 +181,11 @@ class Context:
             _check(lib().ndbx_profile_reset(self._h))
         return out
 
+    def profile_select(self, name=None):
+        """Events only around launches named `name` (None: every launch)."""
+        _check(lib().ndbx_profile_select(self._h, name.encode() if name else None))
+        return self
+
     def close(self):
         if self._h:
             _check(lib().ndbx_destroy(self._h))
@@ -257,6 +263,9 @@ class Batch:
 
     def profile(self, reset=False):
         return Context.profile(self, reset)
+
+    def profile_select(self, name=None):
+        return Context.profile_select(self, name)
 
     def sync(self):
         _check(lib().ndbx_sync(self._h))
