@@ -307,12 +307,15 @@ def main():
                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                         "traffic": traffic_from_profiles(dominant, args.workload),
                         "algorithmic_bytes": ab, "avg_ms": avg_s * 1e3}
-            if dominant in ("count", "brute_count"):
-                # compute view: F = 3d flops per true (ordered) neighbour pair
-                flops = 3.0 * d * float(np.asarray(nn, np.int64).sum())
+            if dominant in ("count", "brute_count") and stats["pair_tests"]:
+                # compute view: the arithmetic issued, d differences, d products
+                # and d - 1 sums per pair test (pairs inside dense cells'
+                # strip rows are counted from tables, without tests)
+                flops = (3.0 * d - 1.0) * float(stats["pair_tests"])
                 tf = flops / avg_s / 1e12
                 roofline["valu"] = {"achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                                    "frac": tf / FP32_PEAK_TFLOPS, "flops": flops}
+                                    "frac": tf / FP32_PEAK_TFLOPS, "flops": flops,
+                                    "basis": "issued pair tests x (3d - 1) flops"}
     cpu = None if (args.no_cpu_baseline or eps_text is None) else cpu_baseline(x, eps_text, m, n)
 
     gnn = extra.pop("graph_nn", None)
