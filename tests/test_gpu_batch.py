@@ -62,3 +62,28 @@ def test_batch_rejects_bad_offsets(gpu_lib):
     with pytest.raises(gpu_lib.NdbxError) as e:
         gpu_lib.Batch(x, np.array([0, 5, 3, 10]), 0.5, 3)
     assert e.value.code == gpu_lib.EINVAL
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_batch_tiny_instances(gpu_lib, d):
+    """Hundreds of instances of 0-40 points: most waves of the per-point
+    passes hold several instances, most count tiles one or two cells."""
+    rng = np.random.default_rng(40 + d)
+    sizes = rng.integers(0, 41, size=300)
+    parts = [rng.normal(0.0, 1.0, size=(int(k), d)).astype(np.float32) * np.float32(rng.uniform(0.2, 2.0))
+             for k in sizes]
+    x = np.concatenate(parts, 0)
+    off = np.zeros(len(sizes) + 1, np.int64)
+    off[1:] = np.cumsum(sizes)
+    eps = rng.uniform(0.2, 1.0, size=len(sizes)).astype(np.float32)
+    m = rng.integers(2, 6, size=len(sizes)).astype(np.int32)
+    print(f"case batch tiny d={d} n={len(x)}", flush=True)
+    with gpu_lib.Batch(x, off, eps, m) as b:
+        b.run()
+        nn, lab, meta = b.nn(), b.labels(), b.meta()
+    for i in range(len(sizes)):
+        s, e = off[i], off[i + 1]
+        nn_o, lab_o, meta_o = O.dbscan(x[s:e], eps[i], m[i])
+        np.testing.assert_array_equal(nn[s:e], nn_o, err_msg=f"instance {i}")
+        np.testing.assert_array_equal(lab[s:e], lab_o, err_msg=f"instance {i}")
+        assert tuple(meta[i]) == meta_o, i

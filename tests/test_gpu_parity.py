@@ -114,6 +114,27 @@ def test_rerun_is_identical(gpu_lib):
     assert a[2] == b[2]
 
 
+def test_profile_select(gpu_lib):
+    """Events around one kernel name only, then every launch again; the
+    results do not depend on what is evented."""
+    x, eps, m = synth.make("blobs3d_10m", n=40000)
+    with gpu_lib.Context(x, eps, m, flags=gpu_lib.PROFILE) as c:
+        c.classify().label()
+        every = c.profile(reset=True)
+        ref = (c.nn(), c.labels(), c.meta())
+        c.profile_select("count").classify().label()
+        only = c.profile(reset=True)
+        got = (c.nn(), c.labels(), c.meta())
+        c.profile_select(None).classify().label()
+        again = c.profile()
+    assert "count" in every and len(every) > 5
+    assert set(only) == {"count"} and only["count"][1] == 1
+    assert set(again) == set(every)
+    np.testing.assert_array_equal(got[0], ref[0])
+    np.testing.assert_array_equal(got[1], ref[1])
+    assert got[2] == ref[2]
+
+
 def test_state_errors(gpu_lib):
     x = np.zeros((10, 2), np.float32)
     c = gpu_lib.Context(x, 0.5, 2)
