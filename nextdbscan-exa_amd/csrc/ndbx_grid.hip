@@ -1008,6 +1008,7 @@ __global__ __launch_bounds__(256) void k_count_sym(const int4 *__restrict__ ctil
                                                    int32_t *__restrict__ nn_s, int64_t *__restrict__ tests,
                                                    const int32_t *__restrict__ part_fine,
                                                    const int32_t *__restrict__ big_id) {
+    __shared__ float4 qsh[4][16];  // per wave: the queries of a small tile
     const int64_t nc = *ncp;
     const int64_t n_tiles = *ntp;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1124,9 +1125,12 @@ __global__ __launch_bounds__(256) void k_count_sym(const int4 *__restrict__ ctil
         issued *= tile.w;
     } else {
         // one candidate per lane, the tile's queries in turn; the tile's own
-        // cells after its first are lane 63's window (lanes >= rows_per_cell
-        // have none)
-        if (lane == 63) {
+        // cells after its first are the window of lane R (the lane after the
+        // stencil rows, which have none), so the windows sit in lanes
+        // [centre, R] and a lane finds its window in log2(kSearch) steps
+        constexpr int R = kStencilRows<D>, C0 = R / 2;
+        constexpr int kSearch = D == 3 ? 16 : D == 2 ? 4 : 2;  // power of two >= R - C0 + 1
+        if (lane == R) {
             ps = m_beg;
             pe = m_end;
         }
@@ -1139,35 +1143,56 @@ __global__ __launch_bounds__(256) void k_count_sym(const int4 *__restrict__ ctil
         }
         const int pre = incl - len;
         const int total = __builtin_amdgcn_readlane(incl, 63);
+        // the queries (coordinates, end of the query's own cell) for broadcast
+        // reads: one 16-byte LDS read per query and round
+        if (lane < 16) qsh[wave][lane] = make_float4(qx[0], D >= 2 ? qx[D >= 2 ? 1 : 0] : 0.0f,
+                                                     D >= 3 ? qx[D >= 3 ? 2 : 0] : 0.0f, __int_as_float(my_end));
+        __builtin_amdgcn_wave_barrier();
         int32_t qcnt[16];
 #pragma unroll
         for (int i = 0; i < 16; ++i) qcnt[i] = 0;
-        for (int cb = 0; cb < total; cb += 64) {
-            const int gi = cb + lane;
-            const bool okc = gi < total;
-            const int gc = okc ? gi : total - 1;  // all lanes take part in the shuffles
-            int lo_w = 0, hi_w = 63;  // last lane whose window starts at or before gc
+        // candidate gc of the flattened windows: its point index
+        auto cand_of = [&](int cb0) {
+            const int gc = cb0 + lane < total ? cb0 + lane : total - 1;  // all lanes take part in the shuffles
+            int lo_w = C0, hi_w = C0 + kSearch - 1;  // last lane whose window starts at or before gc
 #pragma unroll
-            for (int it = 0; it < 6; ++it) {
+            for (int it = 1; it < kSearch; it <<= 1) {
                 const int mid = (lo_w + hi_w + 1) >> 1;
                 if (__shfl(pre, mid, 64) <= gc) lo_w = mid;
                 else hi_w = mid - 1;
             }
-            const int j = __shfl(ps, lo_w, 64) + gc - __shfl(pre, lo_w, 64);
+            return __shfl(ps, lo_w, 64) + gc - __shfl(pre, lo_w, 64);
+        };
+        // rounds of 64 candidates; the next round's point loads are issued
+        // before this round's tests
+        int jn = total > 0 ? cand_of(0) : 0;
+        float cn[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) cn[k] = total > 0 ? soa[k * npad + jn] : 0.0f;
+        for (int cb = 0; cb < total; cb += 64) {
+            const bool okc = cb + lane < total;
+            const int j = jn;
             float c[D];
 #pragma unroll
-            for (int k = 0; k < D; ++k) c[k] = soa[k * npad + j];
+            for (int k = 0; k < D; ++k) c[k] = cn[k];
+            if (cb + 64 < total) {
+                jn = cand_of(cb + 64);
+#pragma unroll
+                for (int k = 0; k < D; ++k) cn[k] = soa[k * npad + jn];
+            }
             int32_t ccount = 0;
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
                 if (i < tile.w) {
+                    const float4 qv = qsh[wave][i];
+                    const float qq[3] = {qv.x, qv.y, qv.z};
                     float l2 = 0.0f;
 #pragma unroll
                     for (int k = 0; k < D; ++k) {
-                        const float df = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qx[k]), i)) - c[k];
+                        const float df = qq[k] - c[k];
                         l2 = __fadd_rn(l2, __fmul_rn(df, df));
                     }
-                    const bool ok = okc && l2 <= e2 && j >= __builtin_amdgcn_readlane(my_end, i);
+                    const bool ok = okc & (l2 <= e2) & (j >= __float_as_int(qv.w));
                     ccount = add_if(ccount, ok);
                     qcnt[i] += __builtin_popcountll(__ballot(ok));
                 }
