@@ -675,7 +675,10 @@ __device__ __forceinline__ int32_t count_block(const float (&qx)[D], const float
 // contraction, '<='.  2-D tiles whose windows reach a dense cell take that
 // cell through its strip table.
 template <int D>
-__global__ __launch_bounds__(256) void k_count(const int4 *__restrict__ ctiles, const int32_t *__restrict__ ntp,
+// 6 waves per SIMD (VGPRs capped at 80, a few spilled): measured on the 100M
+// 2-D workload 705 ms at 5 (the unbounded allocation), 640 at 6, 658 at 7,
+// 669 at 8.
+__global__ __launch_bounds__(256, 6) void k_count(const int4 *__restrict__ ctiles, const int32_t *__restrict__ ntp,
                                                const int32_t *__restrict__ cell_start,
                                                const uint64_t *__restrict__ cell_key,
                                                const InstGrid *__restrict__ ig, int lin_bits,
@@ -997,7 +1000,10 @@ __device__ __forceinline__ int32_t count_run_sym(const float (&qx)[D], const flo
 // on each lane and loop over their queries: the candidate's count stays in
 // its lane and each query's count is a popcount of the compare mask.
 template <int D>
-__global__ __launch_bounds__(256) void k_count_sym(const int4 *__restrict__ ctiles, const int32_t *__restrict__ ntp,
+// 8 waves per SIMD (SGPRs capped, the rest spilled to VGPR lanes): 7 waves
+// left the tiles' load chains exposed (10M 3-D 12.9 -> 12.0 ms, batch 23.9 ->
+// 21.0 ms).
+__global__ __launch_bounds__(256, 8) void k_count_sym(const int4 *__restrict__ ctiles, const int32_t *__restrict__ ntp,
                                                    const int32_t *__restrict__ cell_start,
                                                    const int32_t *__restrict__ cell_of,
                                                    const uint64_t *__restrict__ cell_key,

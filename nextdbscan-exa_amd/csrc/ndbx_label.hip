@@ -694,7 +694,8 @@ __global__ __launch_bounds__(256) void k_border_pts(const int32_t *__restrict__ 
 // Windows are the count windows of the tile, over the box of its searching
 // points.  Each wave takes listed tiles in a grid-stride loop.
 template <int D>
-__global__ __launch_bounds__(256) void k_border(const int4 *__restrict__ ctiles, const int32_t *__restrict__ list,
+// 8 waves per SIMD (SGPRs capped at 8 waves' share).
+__global__ __launch_bounds__(256, 8) void k_border(const int4 *__restrict__ ctiles, const int32_t *__restrict__ list,
                                                 const int32_t *__restrict__ n_list,
                                                 const int32_t *__restrict__ cell_start,
                                                 const uint64_t *__restrict__ cell_key,
