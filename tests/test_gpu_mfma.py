@@ -85,3 +85,16 @@ def test_mfma_guard_band_queue(gpu_lib, monkeypatch, cap):
     np.testing.assert_array_equal(nn, nn_o)
     np.testing.assert_array_equal(lab, lab_o)
     assert meta == meta_o
+
+
+def test_mfma_union_matches_all_pairs_at_scale(gpu_lib):
+    """The 64-D clustered workload at 60K points: counts, labels and the
+    cluster count of the filter path equal the all-pairs path's (an
+    independent union over points)."""
+    x, eps = synth.clusters64d_1m(60000, seed=1003)
+    print("case mfma clusters64d n=60000 vs all-pairs", flush=True)
+    a = gpu_lib.dbscan(x, eps, 15)
+    b = gpu_lib.dbscan(x, eps, 15, flags=gpu_lib.FORCE_BRUTE)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    assert a[2] == b[2]
