@@ -220,26 +220,17 @@ __device__ __forceinline__ bool wave_any_within(const float (&qx)[D], const floa
 // are united.  Either phase alone is correct; phase 0 makes phase 1 cheap.
 constexpr int kMaxStencilCells = 128;  // 25 rows x 5 cells for d <= 3
 
+// One big core cell A = list[li] for one wave (the wave's LDS rows passed in).
 template <int D>
-__global__ __launch_bounds__(256) void k_union(const int32_t *__restrict__ list, const int32_t *__restrict__ n_list,
-                                               const int32_t *__restrict__ ncp, int phase, const int32_t *__restrict__ cell_start,
-                                               const uint64_t *__restrict__ cell_key,
-                                               const InstGrid *__restrict__ ig, int lin_bits,
-                                               const int2 *__restrict__ ranges, int rows_per_cell,
-                                               const int32_t *__restrict__ cell_c,
-                                               const float *__restrict__ core_pts,
-                                               const uint64_t *__restrict__ core_bits,
-                                               int32_t *__restrict__ parent) {
-    const int64_t nc = *ncp;
-    // per wave: candidate cells, their roots, their offsets from A (packed
-    // (dc0 + 2) + 5 (dc1 + 2) + 25 (dc2 + 2)) and point ranges
-    __shared__ int32_t cand[4][kMaxStencilCells];
-    __shared__ int32_t cand_root[4][kMaxStencilCells];
-    __shared__ int32_t cand_off[4][kMaxStencilCells];
-    __shared__ int2 cand_pts[4][kMaxStencilCells];
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t li = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
-    if (li >= *n_list) return;
+__device__ __forceinline__ void union_cell(int64_t li, const int32_t *__restrict__ list, int64_t nc, int phase,
+                                           const int32_t *__restrict__ cell_start,
+                                           const uint64_t *__restrict__ cell_key, const InstGrid *__restrict__ ig,
+                                           int lin_bits, const int2 *__restrict__ ranges, int rows_per_cell,
+                                           const int32_t *__restrict__ cell_c, const float *__restrict__ core_pts,
+                                           const uint64_t *__restrict__ core_bits, int32_t *__restrict__ parent,
+                                           int wave, int32_t (*cand)[kMaxStencilCells],
+                                           int32_t (*cand_root)[kMaxStencilCells],
+                                           int32_t (*cand_off)[kMaxStencilCells], int2 (*cand_pts)[kMaxStencilCells]) {
     const int A = list[li];
     const int lane = threadIdx.x & 63;
     const InstGrid &g = ig[cell_key[A] >> lin_bits];
@@ -385,6 +376,36 @@ __global__ __launch_bounds__(256) void k_union(const int32_t *__restrict__ list,
                 __builtin_amdgcn_wave_barrier();
             }
         }
+    }
+}
+
+// Waves take the listed cells in a grid-stride loop (a fixed grid: launching a
+// wave per grid cell, most of them past the list, cost more than the work on
+// sparse grids).
+template <int D>
+__global__ __launch_bounds__(256) void k_union(const int32_t *__restrict__ list, const int32_t *__restrict__ n_list,
+                                               const int32_t *__restrict__ ncp, int phase, const int32_t *__restrict__ cell_start,
+                                               const uint64_t *__restrict__ cell_key,
+                                               const InstGrid *__restrict__ ig, int lin_bits,
+                                               const int2 *__restrict__ ranges, int rows_per_cell,
+                                               const int32_t *__restrict__ cell_c,
+                                               const float *__restrict__ core_pts,
+                                               const uint64_t *__restrict__ core_bits,
+                                               int32_t *__restrict__ parent) {
+    const int64_t nc = *ncp;
+    // per wave: candidate cells, their roots, their offsets from A (packed
+    // (dc0 + 2) + 5 (dc1 + 2) + 25 (dc2 + 2)) and point ranges
+    __shared__ int32_t cand[4][kMaxStencilCells];
+    __shared__ int32_t cand_root[4][kMaxStencilCells];
+    __shared__ int32_t cand_off[4][kMaxStencilCells];
+    __shared__ int2 cand_pts[4][kMaxStencilCells];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t nl = *n_list;
+    for (int64_t li = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave; li < nl;
+         li += (int64_t)gridDim.x * (blockDim.x >> 6)) {
+        union_cell<D>(li, list, nc, phase, cell_start, cell_key, ig, lin_bits, ranges, rows_per_cell, cell_c,
+                      core_pts, core_bits, parent, wave, cand, cand_root, cand_off, cand_pts);
+        __builtin_amdgcn_wave_barrier();  // the wave's LDS rows are reused by its next cell
     }
 }
 
@@ -938,7 +959,7 @@ static void grid_label_d(Ctx &c) {
             k_compress<<<(unsigned)ceil_div(nc, 256), 256, 0, s>>>(big, counts, c.parent.as<int32_t>());
         }
         Launch L(c.prof, s, phase == 0 ? "union_faces" : "union");
-        k_union<D><<<(unsigned)ceil_div(nc, 4), 256, 0, s>>>(big, counts, ncp, phase, c.cell_start.as<int32_t>(),
+        k_union<D><<<(unsigned)std::min<int64_t>(ceil_div(nc, 4), 4096), 256, 0, s>>>(big, counts, ncp, phase, c.cell_start.as<int32_t>(),
                                                              c.cell_key.as<uint64_t>(), ig, c.lin_bits,
                                                              c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(),
                                                              c.soa_core.as<float>(), c.core_bits.as<uint64_t>(),
