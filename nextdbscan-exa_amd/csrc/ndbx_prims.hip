@@ -267,26 +267,51 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_hist(const uint64_t *__
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
+// Stable scatter of one block's tile by the digit at `shift`.  The tile's
+// keys are first placed in LDS in digit order (a digit's keys keep their
+// input order: rounds, then waves, then lanes), then written out with
+// consecutive threads on consecutive positions, so each digit's run of the
+// tile is one coalesced stretch of the output.  The tile's count of each
+// digit is the difference of consecutive entries of the scanned histogram
+// (digit-major, one entry per block).
 __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ vals, uint64_t *__restrict__ keys_out,
     uint32_t *__restrict__ vals_out, int64_t n, int shift, const int32_t *__restrict__ offs_g,
     int64_t nblocks) {
-    __shared__ int32_t offs[256];
+    __shared__ uint64_t sk[kRadixTile];
+    __shared__ uint32_t sv[kRadixTile];
+    __shared__ int32_t gofs[256], dstart[256], run[256];
     __shared__ int32_t wcnt[kRadixThreads / 64][256];
+    __shared__ int32_t sh_w[kRadixThreads / 64];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
-    offs[threadIdx.x] = offs_g[(int64_t)threadIdx.x * nblocks + blockIdx.x];
     const int64_t base = (int64_t)blockIdx.x * kRadixTile;
+    const int tile_n = (int)(n - base < kRadixTile ? n - base : kRadixTile);
+    // this block's global offset and tile count of digit threadIdx.x
+    const int64_t idx = (int64_t)threadIdx.x * nblocks + blockIdx.x;
+    const int32_t g0 = offs_g[idx];
+    const int32_t g1 = idx + 1 < 256 * nblocks ? offs_g[idx + 1] : (int32_t)n;
+    gofs[threadIdx.x] = g0;
+    int32_t tot;
+    dstart[threadIdx.x] = block_excl_scan(g1 - g0, sh_w, &tot);
+    run[threadIdx.x] = 0;
+    // the tile's keys in registers (all loads in flight together)
+    uint64_t key[kRadixItems];
+    uint32_t val[kRadixItems];
+#pragma unroll
+    for (int r = 0; r < kRadixItems; ++r) {
+        const int li = r * kRadixThreads + threadIdx.x;
+        key[r] = li < tile_n ? keys[base + li] : 0;
+        val[r] = li < tile_n ? vals[base + li] : 0;
+    }
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
     for (int r = 0; r < kRadixItems; ++r) {
 #pragma unroll
         for (int q = 0; q < kRadixThreads / 64; ++q) wcnt[q][threadIdx.x] = 0;
         __syncthreads();
-        const int64_t i = base + (int64_t)r * kRadixThreads + threadIdx.x;
-        const bool valid = i < n;
-        uint64_t key = valid ? keys[i] : 0;
-        uint32_t val = valid ? vals[i] : 0;
-        const unsigned dg = (unsigned)(key >> shift) & 255u;
+        const bool valid = r * kRadixThreads + (int)threadIdx.x < tile_n;
+        const unsigned dg = (unsigned)(key[r] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -300,19 +325,26 @@ __global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(
         if (valid && rank_in_wave == cnt_in_wave - 1) wcnt[w][dg] = cnt_in_wave;
         __syncthreads();
         if (valid) {
-            int pos = offs[dg] + rank_in_wave;
+            int pos = dstart[dg] + run[dg] + rank_in_wave;
             for (int q = 0; q < w; ++q) pos += wcnt[q][dg];
-            keys_out[pos] = key;
-            vals_out[pos] = val;
+            sk[pos] = key[r];
+            sv[pos] = val[r];
         }
         __syncthreads();
         int add = 0;
 #pragma unroll
         for (int q = 0; q < kRadixThreads / 64; ++q) add += wcnt[q][threadIdx.x];
-        offs[threadIdx.x] += add;
-        // the next round's zeroing of wcnt is ordered after this read by the
-        // barrier that follows it
+        run[threadIdx.x] += add;
+        // every wave has read this round's wcnt (positions above) before the
+        // next round clears it
         __syncthreads();
+    }
+    for (int p = threadIdx.x; p < tile_n; p += kRadixThreads) {
+        const uint64_t k = sk[p];
+        const unsigned dg = (unsigned)(k >> shift) & 255u;
+        const int64_t gp = (int64_t)gofs[dg] + (p - dstart[dg]);
+        keys_out[gp] = k;
+        vals_out[gp] = sv[p];
     }
 }
 
