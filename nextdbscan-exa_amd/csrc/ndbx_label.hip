@@ -333,7 +333,8 @@ __device__ __forceinline__ void union_cell(int64_t li, const int32_t *__restrict
     const double r2 = (double)e2 * (1.0 + 3.814697265625e-06);
     int32_t joined_root[4] = {-1, -1, -1, -1};  // roots met earlier in this wave (phase 1)
     int n_joined = 0;
-    for (int q0 = a0; q0 < a1; q0 += 64) {
+    int left = n_cand;  // candidates not yet joined (the walk over A's cores ends at 0)
+    for (int q0 = a0; q0 < a1 && left > 0; q0 += 64) {
         const int q = q0 + lane;
         float qx[D], lo[D], hi[D];
         load_core<D>(core_pts, q < a1 ? q : a0, qx);
@@ -352,7 +353,12 @@ __device__ __forceinline__ void union_cell(int64_t li, const int32_t *__restrict
                 const int rb = __builtin_amdgcn_readfirstlane(cand_root[wave][ci]);
                 bool seen = false;
                 for (int u = 0; u < n_joined && u < 4; ++u) seen |= joined_root[u] == rb;
-                if (seen) continue;
+                if (seen) {  // B's set (as listed) has joined A's: done
+                    if (lane == 0) cand[wave][ci] = -1;
+                    --left;
+                    __builtin_amdgcn_wave_barrier();
+                    continue;
+                }
             }
             // box of this chunk's cores against B's cell box
             const int off = __builtin_amdgcn_readfirstlane(cand_off[wave][ci]);
@@ -372,6 +378,7 @@ __device__ __forceinline__ void union_cell(int64_t li, const int32_t *__restrict
                     cand[wave][ci] = -1;
                     uf_unite(parent, A, B);
                 }
+                --left;
                 if (phase == 1 && n_joined < 4) joined_root[n_joined++] = __builtin_amdgcn_readfirstlane(cand_root[wave][ci]);
                 __builtin_amdgcn_wave_barrier();
             }
