@@ -21,6 +21,10 @@ constexpr int64_t kBoundsChunk = 1 << 13;  // points per bounds workgroup (enoug
 constexpr int kFineBits = 6;  // 2-D: points of a cell in Morton order of a 64 x 64 sub-grid
 constexpr int kPartMin = 64;  // count kernel: cells with fewer points are taken whole, not by parts
 constexpr int kRunGap = 8;    // symmetric count: candidate runs closer than this are walked as one
+// symmetric count: cells of 64+ points are taken by their parts in reach of
+// tile boxes up to this many cells wide (10M 3-D count: 12.07 ms at 1, 11.72
+// at 1.5, 11.59 at 2, 11.90 unbounded; the batch: 21.1, 21.1, 21.2, 23.2 ms)
+constexpr double kPartBoxCells = 2.0;
 
 // ------------------------------------------------------------ bounds --
 __global__ __launch_bounds__(256) void k_bounds(const float *__restrict__ x, int d, int dg,
@@ -996,7 +1000,7 @@ __device__ __forceinline__ int32_t count_run_sym(const float (&qx)[D], const flo
 // One wave per count tile, as k_count, over the higher-index candidates only.
 // Tiles of more than 16 queries walk the windows wave-uniformly (scalar
 // candidate loads, one query per lane; cells of 64+ points by the parts in
-// reach of a tile box no wider than a cell).  Smaller tiles put one candidate
+// reach of a tile box at most kPartBoxCells wide).  Smaller tiles put one candidate
 // on each lane and loop over their queries: the candidate's count stays in
 // its lane and each query's count is a popcount of the compare mask.
 template <int D>
@@ -1060,7 +1064,7 @@ __global__ __launch_bounds__(256, 8) void k_count_sym(const int4 *__restrict__ c
         const double r2o = search_r2(e2);
         bool small_box = part_fine != nullptr;
 #pragma unroll
-        for (int k = 0; k < D; ++k) small_box = small_box && (double)(hi[k] - lo[k]) * g.inv_s <= 1.0;
+        for (int k = 0; k < D; ++k) small_box = small_box && (double)(hi[k] - lo[k]) * g.inv_s <= kPartBoxCells;
         while (rows) {
             const int slot = __builtin_ctzll(rows);
             rows &= rows - 1;
