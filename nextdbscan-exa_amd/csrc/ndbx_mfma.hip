@@ -354,6 +354,10 @@ __device__ __forceinline__ bool mf_takes(int c, int o) { return o == c || ((c < 
 // Rows and columns that take no part are masked by an infinite norm, so the
 // estimate never passes; the few estimates inside the guard band are decided
 // afterwards with the exact predicate, under a ballot.
+// Candidate entries per cell whose same-set state the union's second phase
+// takes at workgroup start (later entries are tested per block only).
+constexpr int kSameSetCap = 4096;
+
 template <int KP, int PASS>
 __global__ __launch_bounds__(256) void k_mfma_pass(MfmaArgs A) {
     constexpr int KS = KP / 16;
@@ -363,6 +367,8 @@ __global__ __launch_bounds__(256) void k_mfma_pass(MfmaArgs A) {
     __shared__ float sb_na[2][32];
     __shared__ int32_t sb_aux[2][32];  // PASS 1: column root; PASS 2: column cluster id
     __shared__ int32_t any_work;
+    __shared__ uint32_t ss_bits[kSameSetCap / 32];
+    __shared__ int32_t ss_pub[2];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     int64_t gidx = blockIdx.x;
     if (PASS == 3) gidx *= A.audit_stride;
@@ -430,21 +436,49 @@ __global__ __launch_bounds__(256) void k_mfma_pass(MfmaArgs A) {
     int ci = c_beg, cb = 0, ce = 0, cs = 0;
     // Union, phase 1: a pair whose cells each have all their cores in one
     // set, the same set, holds no union to make (read-only root walks).
+    // Other workgroups unite sets while this one runs, so the test can turn
+    // true at any moment; every decision that shapes the block sequence is
+    // therefore taken once for the workgroup and read from LDS, never by
+    // each wave for itself (waves that disagreed would walk different block
+    // sequences and fill one LDS block from two).
+    const bool set_skip = PASS == 1 && A.union_phase == 1;
     auto root_of = [&](int32_t x) -> int32_t {
         for (int32_t p = mf_ld(A.parent + x); p != x; p = mf_ld(A.parent + x)) x = p;
         return x;
     };
-    const int32_t rep_c = (PASS == 1 && A.union_phase == 1) ? A.cell_rep[cell] : -1;
-    auto same_set = [&](int o) -> bool {
-        if (PASS != 1 || A.union_phase != 1 || rep_c < 0) return false;
+    const int32_t rep_c = set_skip ? A.cell_rep[cell] : -1;
+    auto same_set_now = [&](int o) -> bool {
+        if (!set_skip || rep_c < 0) return false;
         const int32_t rep_o = A.cell_rep[o];
         return rep_o >= 0 && root_of(rep_c) == root_of(rep_o);
     };
+    // ss_bits: candidate entries (the first kSameSetCap of the cell) whose
+    // pair was one set when the workgroup started; ss_pub[b]: the entry whose
+    // pair was found to be one set while its block was fetched into buffer b,
+    // or -1.
+    if (set_skip) {
+        const int ncand = min(c_end - c_beg, kSameSetCap);
+        for (int base = wave * 64; base < ncand; base += 256) {
+            const int idx = base + lane;
+            const uint64_t m = __ballot(idx < ncand && same_set_now(A.cand[c_beg + idx]));
+            if (lane == 0) {
+                ss_bits[base >> 5] = (uint32_t)m;
+                ss_bits[(base >> 5) + 1] = (uint32_t)(m >> 32);
+            }
+        }
+        if (threadIdx.x < 2) ss_pub[threadIdx.x] = -1;
+        __syncthreads();
+    }
+    auto same_set_at_start = [&](int ci_) -> bool {
+        const int idx = ci_ - c_beg;
+        return set_skip && idx < kSameSetCap && ((ss_bits[idx >> 5] >> (idx & 31)) & 1u);
+    };
+    int buf = 0;  // LDS buffer of the block being computed
     // the candidate cells a pass visits (an empty range for the others)
     auto wanted = [&](int ci_, int o) -> bool {
         if (PASS == 0 || PASS == 3) return true;
         if (PASS == 1 && (o == cell) != (A.union_phase == 0)) return false;
-        if (PASS == 1 && same_set(o)) return false;
+        if (PASS == 1 && same_set_at_start(ci_)) return false;
         if (mf_takes(cell, o)) return A.pflag[ci_] != 0;
         if (PASS == 1) return false;
         int lo_ = A.cand_off[o], hi_ = A.cand_off[o + 1];  // entry of `cell` in o's list
@@ -470,7 +504,7 @@ __global__ __launch_bounds__(256) void k_mfma_pass(MfmaArgs A) {
             if (PASS == 1 && cb_ + 31 < min_row_point && A.cand[ci_] == cell)
                 cb_ += 32 * ((min_row_point - cb_) / 32);
             // a pair whose cells have come to share one set since its first block
-            if (PASS == 1 && cb_ > cs_ && cb_ < ce_ && same_set(A.cand[ci_])) cb_ = ce_;
+            if (set_skip && cb_ > cs_ && cb_ < ce_ && ss_pub[buf] == ci_) cb_ = ce_;
             if (cb_ < ce_) return;
             ++ci_;
             if (ci_ < c_end) load_cell(ci_, cb_, cs_, ce_);
@@ -528,7 +562,6 @@ __global__ __launch_bounds__(256) void k_mfma_pass(MfmaArgs A) {
         publish(0);
     }
     __syncthreads();
-    int buf = 0;
     float mid_r[16];  // PASS 0: (|a|^2 - e2)/2 per row (+inf for rows out of range)
 #pragma unroll
     for (int r = 0; r < 16; ++r) mid_r[r] = 0.5f * (na_r[r] - A.e2);
@@ -661,6 +694,7 @@ __global__ __launch_bounds__(256) void k_mfma_pass(MfmaArgs A) {
             }
         } while (0);
         if (more) publish(buf);
+        if (set_skip && more && threadIdx.x == 0) ss_pub[buf] = same_set_now(A.cand[ci]) ? ci : -1;
         __syncthreads();
     }
     if (!wave_on) return;
