@@ -1061,7 +1061,7 @@ __global__ __launch_bounds__(256, 8) void k_count_sym(const int4 *__restrict__ c
         issued += m_end - m_beg;
         cnt = count_run_sym<D, true>(qw, soa, npad, m_beg, m_end, e2, my_end, lane, nn_s, cnt);
         uint64_t rows = __ballot(pe > ps);
-        const double r2o = search_r2(e2);
+        const double r2o = search_r2(e2), r2in = (double)e2 * (1.0 - 9.5367431640625e-07);
         bool small_box = part_fine != nullptr;
 #pragma unroll
         for (int k = 0; k < D; ++k) small_box = small_box && (double)(hi[k] - lo[k]) * g.inv_s <= kPartBoxCells;
@@ -1073,15 +1073,22 @@ __global__ __launch_bounds__(256, 8) void k_count_sym(const int4 *__restrict__ c
                 // Cells of kPartMin+ points are taken by their quarter-cell
                 // parts (4^D, Morton order) within reach of the tile box:
                 // lane v tests part v.  Dimensions 1..D-1 are the row's.
+                // A part whose farthest point from the box is within
+                // sqrt(e2 (1 - 2^-20)) holds only neighbours of every query:
+                // the float32 sum of squares is within (1 + 5.1 2^-24) of the
+                // exact square distance, so it cannot exceed e2.  Such parts
+                // are counted without tests.
                 const double s_ = 1.0 / g.inv_s, qs = 0.25 * s_, slack = s_ * 9.5367431640625e-07;
-                double gyz = 0.0;
+                double gyz = 0.0, fyz = 0.0;
 #pragma unroll
                 for (int k = 1; k < D; ++k) {
                     const double a0 = g.origin[k] + (double)cell_c[k * nc + a] * s_ + (double)fine_q<D>(lane, k) * qs -
                                       slack;
                     const double a1 = a0 + qs + 2.0 * slack;
                     const double gp = fmax(0.0, fmax(a0 - (double)hi[k], (double)lo[k] - a1));
+                    const double fp = fmax((double)hi[k] - a0, a1 - (double)lo[k]);
                     gyz += gp * gp;
+                    fyz += fp * fp;
                 }
                 int run = cell_start[a];  // pending run of whole cells
                 for (int B = a; B <= e; ++B) {
@@ -1094,13 +1101,32 @@ __global__ __launch_bounds__(256, 8) void k_count_sym(const int4 *__restrict__ c
                     const double a0 = g.origin[0] + (double)cell_c[B] * s_ + (double)fine_q<D>(lane, 0) * qs - slack;
                     const double a1 = a0 + qs + 2.0 * slack;
                     const double gp = fmax(0.0, fmax(a0 - (double)hi[0], (double)lo[0] - a1));
-                    uint64_t mask = __ballot(lane < (1 << (2 * D)) && gyz + gp * gp <= r2o);
+                    const double fp = fmax((double)hi[0] - a0, a1 - (double)lo[0]);
+                    const bool is_part = lane < (1 << (2 * D));
+                    uint64_t mask = __ballot(is_part && gyz + gp * gp <= r2o);
+                    const uint64_t inside = __ballot(is_part && fyz + fp * fp <= r2in) & mask;
                     const int32_t *po = part_fine + (int64_t)big_id[B] * 64;
                     const int32_t b_end = cell_start[B + 1];
-                    // runs of parts in reach; a gap of at most kRunGap points
-                    // is tested rather than split the run (short runs cost
-                    // more per candidate than the few extra tests)
+                    // runs of parts wholly within eps: each query counts the
+                    // run, each point of it gets the tile's query count
+                    for (uint64_t im = inside; im;) {
+                        const int o1 = __builtin_ctzll(im);
+                        const uint64_t gaps = ~im & ~((1ull << o1) - 1ull);
+                        const int o2 = gaps ? __builtin_ctzll(gaps) : 64;
+                        im &= o2 < 64 ? ~((1ull << o2) - 1ull) : 0ull;
+                        const int jj = po[o1];
+                        const int end = o2 < (1 << (2 * D)) ? po[o2] : b_end;
+                        cnt += end - jj;
+                        for (int jb = jj; jb < end; jb += 64)
+                            if (jb + lane < end) atomicAdd(nn_s + jb + lane, tile.w);
+                    }
+                    mask &= ~inside;
+                    // runs of the other parts in reach; a gap of at most
+                    // kRunGap points that holds no counted part is tested
+                    // rather than split the run (short runs cost more per
+                    // candidate than the few extra tests)
                     int pj = -1, pe = -1;
+                    uint64_t seen = 0;  // parts before the end of the pending run
                     while (mask) {
                         const int o1 = __builtin_ctzll(mask);
                         const uint64_t gaps = ~mask & ~((1ull << o1) - 1ull);  // parts >= o1 not in reach
@@ -1108,8 +1134,10 @@ __global__ __launch_bounds__(256, 8) void k_count_sym(const int4 *__restrict__ c
                         mask &= o2 < 64 ? ~((1ull << o2) - 1ull) : 0ull;
                         const int jj = po[o1];
                         const int end = o2 < (1 << (2 * D)) ? po[o2] : b_end;
-                        if (pj >= 0 && jj - pe <= kRunGap) {
+                        const uint64_t upto = o2 < 64 ? (1ull << o2) - 1ull : ~0ull;
+                        if (pj >= 0 && jj - pe <= kRunGap && (inside & ((1ull << o1) - 1ull) & ~seen) == 0) {
                             pe = end;
+                            seen = upto;
                             continue;
                         }
                         if (pj >= 0) {
@@ -1118,6 +1146,7 @@ __global__ __launch_bounds__(256, 8) void k_count_sym(const int4 *__restrict__ c
                         }
                         pj = jj;
                         pe = end;
+                        seen = upto;
                     }
                     if (pj >= 0) {
                         issued += pe - pj;
