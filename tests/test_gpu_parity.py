@@ -165,3 +165,26 @@ def test_tile_shapes_against_oracle(gpu_lib, monkeypatch, d, group_cells):
     np.testing.assert_array_equal(nn, nn_o)
     np.testing.assert_array_equal(lab, lab_o)
     assert meta == meta_o
+
+
+@pytest.mark.parametrize("kind", ["uniform", "lattice"])
+def test_dense_cube_parts_against_oracle(gpu_lib, kind):
+    """3-D cells of hundreds of points, walked by quarter-cell parts: parts
+    wholly within eps of a tile box are counted without tests.  The lattice
+    (spacing eps/8, exact in float32) puts points on part boundaries and pairs
+    at exactly eps along the axes."""
+    eps = synth.eps_f32("1.0")
+    if kind == "uniform":
+        rng = np.random.default_rng(811)
+        x = rng.uniform(0.0, 3.0, size=(30000, 3)).astype(np.float32)
+    else:
+        g = np.arange(25, dtype=np.float32) * np.float32(0.125)
+        x = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3).copy()
+        x = x[np.random.default_rng(812).permutation(len(x))]
+    print(f"case dense cube {kind} n={len(x)}", flush=True)
+    _, nn, lab, meta, _, path = run(gpu_lib, x, eps, 20)
+    nn_o, lab_o, meta_o = O.dbscan(x, eps, 20)
+    assert path == "grid"
+    np.testing.assert_array_equal(nn, nn_o)
+    np.testing.assert_array_equal(lab, lab_o)
+    assert meta == meta_o
