@@ -359,7 +359,7 @@ __device__ __forceinline__ bool mf_takes(int c, int o) { return o == c || ((c < 
 constexpr int kSameSetCap = 4096;
 
 template <int KP, int PASS>
-__global__ __launch_bounds__(256) void k_mfma_pass(MfmaArgs A) {
+__global__ __launch_bounds__(256, (PASS == 1 && KP < 128) ? 3 : 1) void k_mfma_pass(MfmaArgs A) {
     constexpr int KS = KP / 16;
     constexpr int UNITS = 32 * (KP / 8);  // bf16x8 units of one 32-column block, per hi / lo
     constexpr int ROWU = KP / 8 + 1;      // LDS row stride in units: one pad unit breaks bank conflicts
