@@ -31,6 +31,11 @@ sys.path.insert(0, os.path.join(ROOT, "nextdbscan-exa_amd"))
 METRIC = "points/s clustered, plain vs MFMA-filter/batched, exact labels; % HBM/MFMA peak"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+# The 157.3 TFLOP/s vector figure counts packed FP32 (two lanes' worth per
+# instruction); this build compiles with packed FP32 off, so one wave64 FP32
+# instruction per SIMD per issue slot: 256 CUs x 4 SIMDs x 16 lanes x 2
+# flops (FMA) x 2.4 GHz = 78.6 TFLOP/s, i.e. 39.3e12 lane-operations/s.
+FP32_NONPACKED_PEAK_TFLOPS = 78.6
 BF16_DENSE_PEAK_TFLOPS = 2500.0
 DEFAULT_WORKLOAD = "blobs3d_10m"
 
@@ -315,7 +320,12 @@ def main():
                 tf = flops / avg_s / 1e12
                 roofline["valu"] = {"achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                                     "frac": tf / FP32_PEAK_TFLOPS, "flops": flops,
-                                    "basis": "issued pair tests x (3d - 1) flops"}
+                                    "basis": "issued pair tests x (3d - 1) flops",
+                                    # each flop of a test is its own instruction (no FMA: the
+                                    # reference rounds every product and sum), so the ceiling is
+                                    # the non-packed lane-operation rate, half the FMA flop rate
+                                    "peak_nonpacked_ops": FP32_NONPACKED_PEAK_TFLOPS / 2.0,
+                                    "frac_nonpacked_ops": tf / (FP32_NONPACKED_PEAK_TFLOPS / 2.0)}
     cpu = None if (args.no_cpu_baseline or eps_text is None) else cpu_baseline(x, eps_text, m, n)
 
     gnn = extra.pop("graph_nn", None)
