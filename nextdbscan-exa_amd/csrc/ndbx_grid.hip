@@ -1282,16 +1282,48 @@ __global__ __launch_bounds__(256) void k_classify_out(const int32_t *__restrict_
                                                       const int32_t *__restrict__ cell_of,
                                                       const uint64_t *__restrict__ cell_key,
                                                       const InstGrid *__restrict__ ig, int lin_bits, int64_t n,
-                                                      int32_t *__restrict__ nn, uint8_t *__restrict__ core,
-                                                      uint8_t *__restrict__ core_s) {
+                                                      int32_t *__restrict__ nn, uint8_t *__restrict__ core_s) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int32_t v = nn_s[i];
     const bool c = v >= ig[cell_key[cell_of[i]] >> lin_bits].m;  // data_process.cpp:282
-    const uint32_t p = perm[i];
-    nn[p] = v;
-    core[p] = c;
+    nn[perm[i]] = v;
     core_s[i] = c;
+}
+
+// Core flags in input order from the input-order counts (coalesced; a byte
+// scatter through the permutation costs a partial-line write per point).
+// Instances are contiguous input ranges [off[k], off[k+1]): a wave finds the
+// instance of its first point, and its lanes search only when the wave
+// crosses into the next instance.
+__global__ __launch_bounds__(256) void k_core_in(const int32_t *__restrict__ nn, const int64_t *__restrict__ off,
+                                                 const InstGrid *__restrict__ ig, int n_inst, int64_t n,
+                                                 uint8_t *__restrict__ core) {
+    const int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+    if (base >= n) return;
+    const int64_t i = base + (threadIdx.x & 63);
+    int inst = 0;
+    if (n_inst > 1) {
+        int lo = 0, hi = n_inst - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= base) lo = mid;
+            else hi = mid - 1;
+        }
+        inst = lo;
+        const int64_t last = base + 63 < n ? base + 63 : n - 1;
+        if (off[inst + 1] <= last) {
+            const int64_t iv = i < n ? i : n - 1;
+            hi = n_inst - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (off[mid] <= iv) lo = mid;
+                else hi = mid - 1;
+            }
+            inst = lo;
+        }
+    }
+    if (i < n) core[i] = nn[i] >= ig[inst].m;  // data_process.cpp:282
 }
 
 // Part table: part_off[8 c + o] = first point of cell c whose part (top D
@@ -1554,8 +1586,9 @@ static void grid_classify_d(Ctx &c) {
     {
         Launch L(c.prof, s, "classify_out");
         k_classify_out<<<nb, 256, 0, s>>>(c.nn_s.as<int32_t>(), perm, c.cell_of.as<int32_t>(), c.cell_key.as<uint64_t>(),
-                                          ig, c.lin_bits, n, c.nn.as<int32_t>(), c.core.as<uint8_t>(),
-                                          c.core_s.as<uint8_t>());
+                                          ig, c.lin_bits, n, c.nn.as<int32_t>(), c.core_s.as<uint8_t>());
+        k_core_in<<<nb, 256, 0, s>>>(c.nn.as<int32_t>(), c.offsets_d.as<int64_t>(), ig, c.n_inst, n,
+                                     c.core.as<uint8_t>());
     }
     if (stats) {
         unsigned long long h_tests = 0;

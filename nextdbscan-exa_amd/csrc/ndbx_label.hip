@@ -54,6 +54,7 @@ __global__ __launch_bounds__(256) void k_core_points(const uint8_t *__restrict__
                                                      const uint32_t *__restrict__ perm,
                                                      const int32_t *__restrict__ cell_of, int64_t n,
                                                      const int32_t *__restrict__ ncp,
+                                                     const uint8_t *__restrict__ core_in,
                                                      int32_t *__restrict__ core_i32,
                                                      int32_t *__restrict__ cell_core_min) {
     // cells at or past *ncp exist only in a captured run over its capacity
@@ -66,7 +67,7 @@ __global__ __launch_bounds__(256) void k_core_points(const uint8_t *__restrict__
     const int64_t iv = v ? i : wave_base;
     const bool is_core = v && core_s[iv];
     const uint32_t p = perm[iv];
-    if (v) core_i32[p] = is_core;
+    if (v) core_i32[iv] = core_in[iv];  // input order i, coalesced
     const int32_t c = cell_of[iv];
     int32_t val = is_core ? (int32_t)p : INT_MAX;
     // minimum over each run of lanes of one cell (points are sorted by cell):
@@ -926,7 +927,8 @@ static void grid_label_d(Ctx &c) {
     {
         Launch L(c.prof, s, "core");
         k_core_points<<<nb, 256, 0, s>>>(c.core_s.as<uint8_t>(), c.vals.as<uint32_t>(), c.cell_of.as<int32_t>(), n,
-                                         ncp, c.label_s.as<int32_t>(), c.cell_core_min.as<int32_t>());
+                                         ncp, c.core.as<uint8_t>(), c.label_s.as<int32_t>(),
+                                         c.cell_core_min.as<int32_t>());
         k_has_core<<<ncb, 256, 0, s>>>(c.cell_core_min.as<int32_t>(), ncp, c.cell_has_core.as<int32_t>(),
                                        c.core_bits.as<uint64_t>());
     }
