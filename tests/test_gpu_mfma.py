@@ -98,3 +98,19 @@ def test_mfma_union_matches_all_pairs_at_scale(gpu_lib):
     np.testing.assert_array_equal(a[0], b[0])
     np.testing.assert_array_equal(a[1], b[1])
     assert a[2] == b[2]
+
+
+def test_mfma_union_repeatable(gpu_lib):
+    """The union's second launch skips cell pairs whose cores already share a
+    set, a state other workgroups change while it runs; every wave of a
+    workgroup must take the same decision, or one LDS block mixes two column
+    blocks and makes a false union.  Several runs of the 60K-point 64-D case
+    must all give the all-pairs labelling."""
+    x, eps = synth.clusters64d_1m(60000, seed=1003)
+    ref = gpu_lib.dbscan(x, eps, 15, flags=gpu_lib.FORCE_BRUTE)
+    for rep in range(8):
+        print(f"case mfma clusters64d n=60000 run {rep}", flush=True)
+        a = gpu_lib.dbscan(x, eps, 15)
+        np.testing.assert_array_equal(a[0], ref[0])
+        np.testing.assert_array_equal(a[1], ref[1])
+        assert a[2] == ref[2]
