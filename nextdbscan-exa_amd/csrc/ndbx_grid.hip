@@ -1362,16 +1362,20 @@ __global__ __launch_bounds__(256) void k_big_flag(const int32_t *__restrict__ ce
 
 // One thread per point of a qualifying cell: the parts after the previous
 // point's, up to its own, start at it (the last point also closes the
-// parts after its own at the cell end).
+// parts after its own at the cell end).  Cells at or past *ncp exist only in
+// a captured run over its cell capacity (big_id holds the capacity's cells;
+// the run is flagged for an eager rerun): their points are left out.
 template <int D>
 __global__ __launch_bounds__(256) void k_part_fine(const uint64_t *__restrict__ keys, int sub_bits,
                                                    const int32_t *__restrict__ cell_of,
                                                    const int32_t *__restrict__ cell_start,
+                                                   const int32_t *__restrict__ ncp,
                                                    const int32_t *__restrict__ big_id, int64_t n,
                                                    int32_t *__restrict__ part_fine) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int32_t c = cell_of[p];
+    if (c >= *ncp) return;
     const int32_t b = cell_start[c], e = cell_start[c + 1];
     if (e - b < kPartMin) return;
     const int shift = sub_bits - 2 * D;
@@ -1481,7 +1485,7 @@ static void grid_classify_d(Ctx &c) {
         {
             Launch L(c.prof, s, "part_off");
             k_part_fine<D><<<nb, 256, 0, s>>>(keys, c.sub_bits, c.cell_of.as<int32_t>(), c.cell_start.as<int32_t>(),
-                                             c.big_id.as<int32_t>(), n, c.part_fine.as<int32_t>());
+                                             ncp, c.big_id.as<int32_t>(), n, c.part_fine.as<int32_t>());
         }
         part_fine = c.part_fine.as<int32_t>();
         big_id = c.big_id.as<int32_t>();

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/ndbx.h"
+#include "ndbx_limits.h"
 
 namespace ndbx {
 
@@ -112,6 +113,8 @@ void radix_sort_u64(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t
                     int64_t n, int nbits, DevBuf &scratch, hipStream_t s, Profiler &prof,
                     bool *result_in_alt);
 void fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s, Profiler &prof);
+// *out (device, 64-bit) = sum of n int32 values.
+void sum_i32_u64(const int32_t *v, int64_t n, unsigned long long *out, hipStream_t s, Profiler &prof);
 void fill_u64(uint64_t *p, uint64_t v, int64_t n, hipStream_t s, Profiler &prof);
 
 // ------------------------------------------------------------- context --

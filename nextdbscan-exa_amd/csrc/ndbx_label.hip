@@ -593,12 +593,18 @@ __global__ __launch_bounds__(256) void k_cell_label(const int32_t *__restrict__ 
 //                  tiles would leave most of a wave idle, and a wave has many
 //                  more lane searches than tile searches in flight.
 constexpr int kBorderLaneTile = 16;
+// Cells at or past *ncp exist only in a captured run over its cell capacity
+// (cell_label holds the capacity's cells; the run is flagged for an eager
+// rerun): their points get NO_CLUSTER.
 __global__ __launch_bounds__(256) void k_border_own(const int32_t *__restrict__ cell_of,
+                                                    const int32_t *__restrict__ ncp,
                                                     const int32_t *__restrict__ cell_label,
                                                     const uint32_t *__restrict__ perm, int64_t n,
                                                     int32_t *__restrict__ labels) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) labels[perm[i]] = cell_label[cell_of[i]];  // NO_CLUSTER when the cell has no core
+    if (i >= n) return;
+    const int32_t c = cell_of[i];
+    labels[perm[i]] = c < *ncp ? cell_label[c] : NDBX_NO_CLUSTER;  // NO_CLUSTER also when the cell has no core
 }
 
 constexpr int kBorderListThreads = 1024;
@@ -997,7 +1003,7 @@ static void grid_label_d(Ctx &c) {
         c.border_tiles.ensure((size_t)(std::max<int64_t>(nct, 1) + n) * 4);
         int32_t *blist = c.border_tiles.as<int32_t>(), *n_blist = counts + 2;
         int32_t *plist = blist + std::max<int64_t>(nct, 1), *n_plist = counts + 3;
-        k_border_own<<<nb, 256, 0, s>>>(c.cell_of.as<int32_t>(), c.cell_label.as<int32_t>(), c.vals.as<uint32_t>(),
+        k_border_own<<<nb, 256, 0, s>>>(c.cell_of.as<int32_t>(), ncp, c.cell_label.as<int32_t>(), c.vals.as<uint32_t>(),
                                         n, c.labels.as<int32_t>());
         k_border_list<<<(unsigned)ceil_div(std::max<int64_t>(nct, 1), kBorderListThreads), kBorderListThreads, 0, s>>>(
             c.ctiles.as<int4>(), ntp, c.core_s.as<uint8_t>(), c.nn_s.as<int32_t>(), blist, n_blist, plist, n_plist);

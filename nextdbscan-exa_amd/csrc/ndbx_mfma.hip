@@ -1353,6 +1353,19 @@ void mfma_classify(Ctx &c) {
                                                   reinterpret_cast<double *>(c.b_rootmin.p), nc, d, r, nullptr,
                                                   c.nranges.as<int32_t>(), 0, nullptr, nullptr);
     }
+    // the pair list has int32 offsets: past mfma_safe_cells() cells the total
+    // can exceed them, so it is summed in 64 bits and checked first
+    if (nc > mfma_safe_cells()) {
+        unsigned long long *d_sum = reinterpret_cast<unsigned long long *>(c.misc.as<int32_t>() + 624);
+        sum_i32_u64(c.nranges.as<int32_t>(), nc, d_sum, s, c.prof);
+        unsigned long long h_sum = 0;
+        NDBX_HIP(hipMemcpyAsync(&h_sum, d_sum, 8, hipMemcpyDeviceToHost, s));
+        c.sync();
+        if (!mfma_pairs_fit((int64_t)h_sum))
+            throw Error(NDBX_EUNSUPPORTED, "MFMA filter: " + std::to_string(h_sum) + " candidate cell pairs exceed "
+                                           "the int32 pair list (" + std::to_string(kMfmaMaxPairs) +
+                                           "); use a smaller eps or fewer points per run");
+    }
     scan_exclusive_i32(c.nranges.as<int32_t>(), c.tile_off.as<int32_t>(), nc, c.tile_off.as<int32_t>() + nc,
                        c.scratch, s, c.prof);
     int32_t h_pairs = 0;

@@ -117,6 +117,27 @@ void fill_u64(uint64_t *p, uint64_t v, int64_t n, hipStream_t s, Profiler &prof)
     k_fill<uint64_t><<<grid_for(n, 256), 256, 0, s>>>(p, v, n);
 }
 
+// Sum of int32 values into a 64-bit counter (totals that may exceed int32):
+// one atomic per block.
+__global__ __launch_bounds__(256) void k_sum_i32_u64(const int32_t *__restrict__ v, int64_t n,
+                                                     unsigned long long *__restrict__ out) {
+    __shared__ int64_t sh[4];
+    int64_t acc = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        acc += v[i];
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, (unsigned long long)(sh[0] + sh[1] + sh[2] + sh[3]));
+}
+
+void sum_i32_u64(const int32_t *v, int64_t n, unsigned long long *out, hipStream_t s, Profiler &prof) {
+    fill_u64(reinterpret_cast<uint64_t *>(out), 0, 1, s, prof);
+    if (n <= 0) return;
+    Launch L(prof, s, "sum");
+    k_sum_i32_u64<<<grid_for(n, 256, 1024), 256, 0, s>>>(v, n, out);
+}
+
 // -------------------------------------------------------------- scan --
 // Three-phase exclusive scan: per-block sums, scan of the sums (recursive),
 // per-block downsweep seeded with the block's prefix.  Tile = 256 x 8.

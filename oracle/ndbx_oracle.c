@@ -54,13 +54,6 @@ static inline int oracle_valid_source(const float *p) { return p[0] != FLT_MAX; 
 /* data_process.h:73 */
 float oracle_e2(float e) { return e * e; }
 
-/* Pairwise predicate for n pairs (a[i], b[i]); used by the FP fixtures. */
-void oracle_pred_pairs(const float *a, const float *b, int64_t n, int d,
-                       float e2, uint8_t *out) {
-    for (int64_t i = 0; i < n; ++i)
-        out[i] = (uint8_t)oracle_is_neighbour(a + i * d, b + i * d, d, e2);
-}
-
 /* data_process.cpp:395-458: nn[i] = #{ j : valid_source(j) && pred(i, j) }.
  * The reference pushes from chunk point j with atomic adds; the sum is the
  * same integer, so this pulls per i. */
@@ -75,6 +68,53 @@ void oracle_classify(const float *x, int64_t n, int d, float e2, int32_t *nn) {
             if (oracle_is_neighbour(p1, p2, d, e2)) ++hits;
         }
         nn[i] = hits;
+    }
+}
+
+/* oracle_classify restricted to the listed rows: nn[r] for point rows[r]
+ * against every point (data_process.cpp:395-458), for checking a sample of a
+ * run too large for the full O(n^2) count. */
+void oracle_classify_rows(const float *x, int64_t n, int d, float e2,
+                          const int64_t *rows, int64_t n_rows, int32_t *nn) {
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t r = 0; r < n_rows; ++r) {
+        const float *p1 = x + rows[r] * d;
+        int32_t hits = 0;
+        for (int64_t j = 0; j < n; ++j) {
+            const float *p2 = x + j * d;
+            if (!oracle_valid_source(p2)) continue;
+            if (oracle_is_neighbour(p1, p2, d, e2)) ++hits;
+        }
+        nn[r] = hits;
+    }
+}
+
+/* For each listed row: the smallest and largest labels[j] over core points j
+ * (core[j] != 0, valid source) within eps of point rows[r]; INT32_MAX and
+ * INT32_MIN when there is none.  Given a labelling of the cores, the
+ * reference's rules say (data_process.cpp:177-260): a core's own label is
+ * both (all cores within eps of a core share its cluster), and a non-core
+ * point's label is the smallest (the last sweep's atomic_min over its cores,
+ * :204-209), NO_CLUSTER when there is none. */
+void oracle_core_label_range_rows(const float *x, int64_t n, int d, float e2,
+                                  const uint8_t *core, const int32_t *labels,
+                                  const int64_t *rows, int64_t n_rows,
+                                  int32_t *lo, int32_t *hi) {
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t r = 0; r < n_rows; ++r) {
+        const float *p1 = x + rows[r] * d;
+        int32_t a = INT32_MAX, b = INT32_MIN;
+        for (int64_t j = 0; j < n; ++j) {
+            if (!core[j]) continue;
+            const float *p2 = x + j * d;
+            if (!oracle_valid_source(p2)) continue;
+            if (oracle_is_neighbour(p1, p2, d, e2)) {
+                if (labels[j] < a) a = labels[j];
+                if (labels[j] > b) b = labels[j];
+            }
+        }
+        lo[r] = a;
+        hi[r] = b;
     }
 }
 

@@ -32,8 +32,11 @@ def lib():
         u8p = ctypes.POINTER(ctypes.c_uint8)
         L.oracle_e2.restype = ctypes.c_float
         L.oracle_e2.argtypes = [ctypes.c_float]
-        L.oracle_pred_pairs.argtypes = [f32p, f32p, ctypes.c_int64, ctypes.c_int,
-                                        ctypes.c_float, u8p]
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        L.oracle_classify_rows.argtypes = [f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_float,
+                                           i64p, ctypes.c_int64, i32p]
+        L.oracle_core_label_range_rows.argtypes = [f32p, ctypes.c_int64, ctypes.c_int, ctypes.c_float,
+                                                   u8p, i32p, i64p, ctypes.c_int64, i32p, i32p]
         L.oracle_classify.argtypes = [f32p, ctypes.c_int64, ctypes.c_int,
                                       ctypes.c_float, i32p]
         L.oracle_labels.restype = ctypes.c_int32
@@ -53,13 +56,30 @@ def e2(eps) -> np.float32:
     return np.float32(lib().oracle_e2(float(np.float32(eps))))
 
 
-def pred_pairs(a, b, e2v):
-    a = np.ascontiguousarray(a, np.float32)
-    b = np.ascontiguousarray(b, np.float32)
-    out = np.zeros(a.shape[0], np.uint8)
-    lib().oracle_pred_pairs(_p(a, ctypes.c_float), _p(b, ctypes.c_float), a.shape[0],
-                            a.shape[1], float(e2v), _p(out, ctypes.c_uint8))
-    return out.astype(bool)
+def classify_rows(x, eps, rows):
+    """nn of the listed rows only (each against every point)."""
+    x = np.ascontiguousarray(x, np.float32)
+    rows = np.ascontiguousarray(rows, np.int64)
+    nn = np.zeros(len(rows), np.int32)
+    lib().oracle_classify_rows(_p(x, ctypes.c_float), x.shape[0], x.shape[1], float(e2(eps)),
+                               _p(rows, ctypes.c_int64), len(rows), _p(nn, ctypes.c_int32))
+    return nn
+
+
+def core_label_range_rows(x, eps, core, labels, rows):
+    """(smallest, largest) label over the cores within eps of each listed row;
+    (INT32_MAX, INT32_MIN) where there is none."""
+    x = np.ascontiguousarray(x, np.float32)
+    core = np.ascontiguousarray(core, np.uint8)
+    labels = np.ascontiguousarray(labels, np.int32)
+    rows = np.ascontiguousarray(rows, np.int64)
+    lo = np.zeros(len(rows), np.int32)
+    hi = np.zeros(len(rows), np.int32)
+    lib().oracle_core_label_range_rows(_p(x, ctypes.c_float), x.shape[0], x.shape[1], float(e2(eps)),
+                                       _p(core, ctypes.c_uint8), _p(labels, ctypes.c_int32),
+                                       _p(rows, ctypes.c_int64), len(rows), _p(lo, ctypes.c_int32),
+                                       _p(hi, ctypes.c_int32))
+    return lo, hi
 
 
 def classify(x, eps):

@@ -122,3 +122,35 @@ def test_graph_2d_batch(gpu_lib):
             b.run()
             check_same((b.nn(), b.labels(), b.meta()), want, f"2d run {r}")
         assert b.info()[1] == 2
+
+
+def test_graph_over_capacity_dense_cells_past_cap(gpu_lib):
+    """A replay whose cell count outgrows the capacity while the cells past
+    the capacity are dense (64+ points, so they have quarter-cell part
+    tables) must stay in bounds, flag the run and give the eager result.
+    Instance 0 spans [0, 200]^3 from the start, so the key layout is wide
+    enough for the later spread; instances 1-8 go from [0, 4]^3 (few cells)
+    to [0, 160]^3 (one cell per point); the last instance stays a dense cube
+    and its cells (numbered last) land past the capacity."""
+    rng = np.random.default_rng(107)
+    parts = [rng.uniform(0.0, 200.0, size=(500, 3))]
+    parts += [rng.uniform(0.0, 4.0, size=(3000, 3)) for _ in range(8)]
+    parts.append(rng.uniform(0.0, 2.0, size=(8000, 3)))
+    sizes = [len(p) for p in parts]
+    x = np.concatenate(parts).astype(np.float32)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    eps = np.full(len(sizes), 1.0, np.float32)
+    m = np.full(len(sizes), 8, np.int32)
+    x2 = x.copy()
+    x2[off[1]:off[9]] *= np.float32(40.0)
+    print("case graph over capacity, dense cells past the cap", flush=True)
+    with gpu_lib.Batch(x, off, eps, m, gpu_lib.GRAPH) as b:
+        b.run()
+        b.run()
+        check_same((b.nn(), b.labels(), b.meta()), eager(gpu_lib, x, off, eps, m), "compact")
+        b.set_coords(x2).run()
+        check_same((b.nn(), b.labels(), b.meta()), eager(gpu_lib, x2, off, eps, m), "spread")
+        _, replays, reruns = b.info()
+        assert replays == 1 and reruns == 1, (replays, reruns)
+        b.run()
+        check_same((b.nn(), b.labels(), b.meta()), eager(gpu_lib, x2, off, eps, m), "spread, recaptured")

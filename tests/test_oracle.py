@@ -166,3 +166,53 @@ def test_oracle_matches_reference_build_live():
     np.testing.assert_array_equal(nn, rnn)
     np.testing.assert_array_equal(lab, rcci)
     assert meta == rmeta
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_classify_rows_matches_reference_fixture(name):
+    """The row-subset count equals the reference's nn on every fixture, in
+    any row order and with repeated rows."""
+    g = load_golden(name)
+    eps = synth.eps_f32(g["eps_text"])
+    n = len(g["nn"])
+    rows = np.random.default_rng(5).permutation(n)
+    rows = np.concatenate([rows, rows[: min(n, 3)]])
+    np.testing.assert_array_equal(O.classify_rows(g["x"], eps, rows), g["nn"][rows])
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_core_label_range_rows_matches_reference_fixture(name):
+    """On the reference's own labelling: every core's label is both the
+    smallest and the largest label among the cores within eps of it, and a
+    non-core point's label is the smallest (NO_CLUSTER when there is none)."""
+    g = load_golden(name)
+    eps = synth.eps_f32(g["eps_text"])
+    core = g["nn"] >= g["m"]
+    lab = g["cci"]
+    rows = np.arange(len(lab))
+    lo, hi = O.core_label_range_rows(g["x"], eps, core, lab, rows)
+    np.testing.assert_array_equal(lo[core], lab[core])
+    np.testing.assert_array_equal(hi[core], lab[core])
+    np.testing.assert_array_equal(lo[~core], lab[~core])
+    assert (hi[~core & (lab == O.NO_CLUSTER)] == np.iinfo(np.int32).min).all()
+
+
+def test_core_label_range_rows_sees_planted_mistakes():
+    """Checks built on the range function catch a split cluster (one core
+    relabelled) and a wrong border (a border given a larger adjacent id)."""
+    g = load_golden("blobs2d_3k")
+    eps = synth.eps_f32(g["eps_text"])
+    core = g["nn"] >= g["m"]
+    rows = np.arange(len(core))
+    bad = g["cci"].copy()
+    victim = np.nonzero(core & (g["nn"] > g["m"] + 2))[0][0]
+    bad[victim] = bad[victim] + 1
+    lo, hi = O.core_label_range_rows(g["x"], eps, core, bad, rows)
+    assert ((lo[core] != bad[core]) | (hi[core] != bad[core])).any()
+    # two clusters meeting at a border point: give it the larger id
+    lo0, hi0 = O.core_label_range_rows(g["x"], eps, core, g["cci"], rows)
+    amb = np.nonzero(~core & (lo0 != hi0) & (hi0 != np.iinfo(np.int32).min))[0]
+    if len(amb):
+        bad2 = g["cci"].copy()
+        bad2[amb[0]] = hi0[amb[0]]
+        assert (lo0[~core] != bad2[~core]).any()
