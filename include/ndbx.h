@@ -168,6 +168,34 @@ int ndbx_batch_set_coords(ndbx_ctx *ctx, const float *coords);
 int ndbx_batch_info(ndbx_ctx *ctx, int *group_cells, int64_t *graph_replays,
                     int64_t *graph_reruns);
 
+/* Largest input of ndbx_dbscan_noalloc, and the fixed device memory it
+ * works in (held on each device the library runs on, until the process ends). */
+#define NDBX_NOALLOC_MAX_POINTS 131072
+#define NDBX_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/*
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): classify + label of one instance, as ndbx_create with
+ * NDBX_DEVICE_INPUT + ndbx_classify + ndbx_label + ndbx_result_meta, with
+ * every buffer beyond the caller's inputs and outputs taken from the
+ * library's fixed device array of NDBX_NOALLOC_SCRATCH_BYTES; no hipMalloc,
+ * hipFree or other allocating call is made.
+ *   coords  n*d float32 AoS, device memory
+ *   nn      n int32, device memory: neighbour counts (as ndbx_get_nn)
+ *   labels  n int32, device memory: cluster index per point (as ndbx_get_labels)
+ *   meta3   host: (cores, noise, clusters)
+ *   stream  a hipStream_t to run on, or NULL for the null stream; the call
+ *           returns after that stream is idle
+ * Needs 1 <= d <= 3 and 0 <= n <= NDBX_NOALLOC_MAX_POINTS (else NDBX_EINVAL);
+ * flags may hold NDBX_APPROX and NDBX_FORCE_BRUTE only.  Every call on one
+ * device shares that device's array, which keeps the last call's contents:
+ * calls that overlap on one device (two threads, or two streams) race.
+ */
+int ndbx_dbscan_noalloc(const float *coords, int64_t n, int d, float eps, int min_pts,
+                        uint32_t flags, void *stream, int32_t *nn, int32_t *labels,
+                        int32_t *meta3);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 #include <new>
 
 #include "ndbx_internal.h"
@@ -27,7 +28,7 @@ Ctx::~Ctx() {
                       &b_rootmin, &soa_core, &cell_c, &gflag, &ctile_off, &ctiles, &tile_tests, &dense_id, &dense_info, &dense_tmp, &fine_off, &dcount, &plan_in, &part_off,
                       &pflag, &bq, &cgroups, &part_fine, &big_id, &core_bits, &border_tiles};
     for (auto *b : bufs) b->release();
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream && own_stream) (void)hipStreamDestroy(stream);
 }
 
 template <class F>
@@ -227,6 +228,35 @@ static void batch_run_graph(Ctx &c) {
     c.capacity_mode = false;
     c.prof.on = prof_on;
 }
+
+// ----------------------------------------------------- no-alloc calls --
+// The fixed device array behind ndbx_dbscan_noalloc.  Host code never uses
+// its name as an address (on the host it is a stand-in, not device memory):
+// the device address comes from hipGetSymbolAddress, taken once per device
+// with that device current, and only that address reaches kernels and copies.
+static __device__ __align__(256) unsigned char scratch[NDBX_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static std::vector<char *> g_scratch_ptr;  // per device; null until set up
+
+static char *scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        NDBX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw Error(NDBX_EDEVICE, "no-alloc scratch: hipGetSymbolAddress gave a null address");
+        g_scratch_ptr[device] = static_cast<char *>(p);
+    }
+    return g_scratch_ptr[device];
+}
+
+// Points DevBuf::ensure at an arena on this thread for one call.
+struct ArenaScope {
+    Arena *prev;
+    explicit ArenaScope(Arena *a) : prev(g_arena) { g_arena = a; }
+    ~ArenaScope() { g_arena = prev; }
+};
 
 }  // namespace ndbx
 
@@ -489,6 +519,47 @@ int ndbx_audit(ndbx_ctx *h, int64_t *band_pairs, int64_t *fp64_flips, int64_t *f
 
 int ndbx_destroy(ndbx_ctx *h) {
     return guarded([&] { delete h; });
+}
+
+int ndbx_dbscan_noalloc(const float *coords, int64_t n, int d, float eps, int min_pts, uint32_t flags,
+                        void *stream, int32_t *nn, int32_t *labels, int32_t *meta3) {
+    return guarded([&] {
+        need(n >= 0, "n < 0");
+        need(n <= NDBX_NOALLOC_MAX_POINTS, ("n = " + std::to_string(n) + " exceeds NDBX_NOALLOC_MAX_POINTS (" +
+                                            std::to_string(NDBX_NOALLOC_MAX_POINTS) + ")").c_str());
+        need(d >= 1 && d <= 3, "ndbx_dbscan_noalloc takes 1 <= d <= 3");
+        need((flags & ~(uint32_t)(NDBX_APPROX | NDBX_FORCE_BRUTE)) == 0,
+             "ndbx_dbscan_noalloc accepts only NDBX_APPROX and NDBX_FORCE_BRUTE");
+        need(meta3 != nullptr, "meta3 is null");
+        need(n == 0 || (coords && nn && labels), "null coords, nn or labels");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw Error(NDBX_EDEVICE, "no HIP device");
+        int dev = 0;
+        NDBX_HIP(hipGetDevice(&dev));
+        // the array's last MiB stays unused, so a vector load that reads a few
+        // bytes past the last buffer still reads inside the array
+        Arena arena{scratch_ptr(dev), (size_t)NDBX_NOALLOC_SCRATCH_BYTES - (1u << 20), 0};
+        ArenaScope scope(&arena);
+        Ctx c;
+        c.device = dev;
+        c.stream = static_cast<hipStream_t>(stream);
+        c.own_stream = false;
+        c.d = d;
+        c.n = n;
+        c.n_inst = 1;
+        c.h_offsets = {0, n};
+        c.h_eps = {eps};
+        c.h_m = {min_pts};
+        c.flags = flags | NDBX_DEVICE_INPUT;
+        c.coords = coords;
+        c.nn.borrow(nn, (size_t)n * 4);
+        c.labels.borrow(labels, (size_t)n * 4);
+        c.core.ensure((size_t)n);
+        c.meta.ensure(3 * 4);
+        run_classify(c);
+        run_label(c);
+        std::memcpy(meta3, c.h_meta.data(), 3 * sizeof(int32_t));
+    });
 }
 
 }  // extern "C"

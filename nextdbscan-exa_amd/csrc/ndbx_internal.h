@@ -38,13 +38,30 @@ struct Error : std::exception {
 // have to grow then raises an error instead of allocating.
 extern thread_local bool g_no_alloc;
 
+// A fixed device region that buffers are carved from instead of allocated
+// (bump allocation, 256-byte aligned): set on a thread for the duration of a
+// call that must not allocate.  Running past its end raises NDBX_ENOMEM.
+struct Arena {
+    char *base = nullptr;
+    size_t size = 0, used = 0;
+};
+extern thread_local Arena *g_arena;
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool borrowed = false;  // memory not owned (arena or caller's): never freed
     template <class T>
     T *as() const { return static_cast<T *>(p); }
     void *ensure(size_t want);
     void release();
+    // Use caller-owned device memory of `bytes` bytes in place.
+    void borrow(void *mem, size_t nbytes) {
+        release();
+        p = mem;
+        bytes = nbytes;
+        borrowed = true;
+    }
 };
 
 // ------------------------------------------------------------- profiler --
@@ -123,6 +140,7 @@ enum Path { PATH_GRID = 0, PATH_BRUTE = 1, PATH_MFMA = 2 };
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool own_stream = true;  // false: the caller's stream, not destroyed with the context
     int64_t n = 0;
     int d = 0;
     uint32_t flags = 0;

@@ -6,10 +6,25 @@ namespace ndbx {
 
 // ----------------------------------------------------------- buffers --
 thread_local bool g_no_alloc = false;
+thread_local Arena *g_arena = nullptr;
 
 void *DevBuf::ensure(size_t want) {
     if (want == 0) want = 256;
     if (want <= bytes) return p;
+    if (g_arena) {
+        // carve from the fixed region; a buffer that grows takes a new
+        // piece (the old one is not reused within the call)
+        const size_t at = (g_arena->used + 255) & ~(size_t)255;
+        if (at > g_arena->size || want > g_arena->size - at)
+            throw Error(NDBX_ENOMEM, "no-alloc scratch exhausted: " + std::to_string(at + want) + " bytes needed of " +
+                                         std::to_string(g_arena->size));
+        release();
+        p = g_arena->base + at;
+        bytes = want;
+        borrowed = true;
+        g_arena->used = at + want;
+        return p;
+    }
     if (g_no_alloc) throw Error(NDBX_EDEVICE, "device buffer would grow inside a graph capture");
     release();
     NDBX_HIP(hipMalloc(&p, want));
@@ -18,9 +33,10 @@ void *DevBuf::ensure(size_t want) {
 }
 
 void DevBuf::release() {
-    if (p) (void)hipFree(p);
+    if (p && !borrowed) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    borrowed = false;
 }
 
 // ---------------------------------------------------------- profiler --

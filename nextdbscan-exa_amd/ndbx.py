@@ -62,7 +62,10 @@ SIGNATURES = {
     "ndbx_batch_meta": ([_P, _P], ctypes.c_int),
     "ndbx_batch_set_coords": ([_P, _P], ctypes.c_int),
     "ndbx_batch_info": ([_P, _IP, _I64P, _I64P], ctypes.c_int),
+    "ndbx_dbscan_noalloc": ([_P, _I64, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_uint32, _P, _P, _P,
+                             _P], ctypes.c_int),
 }
+NOALLOC_MAX_POINTS = 131072
 
 
 class NdbxError(RuntimeError):
