@@ -5,21 +5,30 @@ A step is one full pass of the path over the resident input: classify
 ending with the (cores, noise, clusters) triple on the host; coordinates are
 in HBM before the timed region starts and per-point results stay in HBM.
 
-    python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--workload NAME] [--headline-only]
+                    [--no-cpu-baseline]
 
-Prints one JSON line.  `value` is points/s = N / step time.  `roofline` is
-computed from HIP events recorded on the library's stream around every
-launch of the dominant kernel in the timed steps.  `cpu_baseline` times the
-reference's own OpenMP process_points (oracle/_ref/ref_harness) on a bounded
-sample of chunk rows of the same input and extrapolates to all N*N pair tests
-of its Classify phase; its Assign-Labels phase is not included, so the CPU
-figure is an upper bound on the reference's points/s.
+Prints one JSON line.  `value` is points/s = N / step time on the headline
+workload (default: 10M points, d = 3).  `roofline` is computed from HIP
+events recorded on the library's stream around every launch of the dominant
+kernel in the timed steps; the count kernel is bound by FP32 vector issue
+(each pair test is separately rounded sub/mul/add, no FMA), so its binding
+roof is the VALU one, with algorithmic flops 3d per ordered neighbour pair
+(3d * sum(nn)); its HBM fraction and issued-test arithmetic are reported
+beside it.  `cpu_baseline` times the reference's own OpenMP build
+(oracle/_ref) on the box's host cores, on a bounded sample of the same
+workload.  `forms` holds the other workloads, each with the form it is
+compared against: the 100k 2-D blobs (eager and graph replay), the 1M 64-D
+clusters (MFMA filter vs the all-pairs float32 kernels, with the float64
+audit), the 4,096-instance batch (batched eager and graph replay vs one
+context per instance) and the skewed 100M 2-D set.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import subprocess
 import sys
 import tempfile
@@ -38,50 +47,98 @@ FP32_PEAK_TFLOPS = 157.3
 FP32_NONPACKED_PEAK_TFLOPS = 78.6
 BF16_DENSE_PEAK_TFLOPS = 2500.0
 DEFAULT_WORKLOAD = "blobs3d_10m"
+EPS_TEXT = {"blobs2d_100k": "0.5", "blobs3d_10m": "1.5", "skew2d_100m": "0.3"}
+REF_HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+REF_CLI = os.path.join(ROOT, "oracle", "_ref", "nextdbscan-omp")
 
 
+# ------------------------------------------------------------ host facts --
+def host_cpu():
+    """Cores this process may use, and the host's socket/core layout."""
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for key, name in (("Socket(s)", "sockets"), ("Core(s) per socket", "cores_per_socket"),
+                          ("Thread(s) per core", "threads_per_core"), ("Model name", "model")):
+            m = re.search(rf"^{re.escape(key)}:\s*(.+)$", out, re.M)
+            if m:
+                v = m.group(1).strip()
+                info[name] = int(v) if v.isdigit() else v
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def physical_cores(info):
+    """Physical cores available to this process (affinity / threads per core)."""
+    return max(1, info["affinity"] // max(1, int(info.get("threads_per_core", 1) or 1)))
+
+
+def omp_env(threads):
+    return dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+
+
+def write_bin(path, x):
+    import synth
+    synth.write_bin(path, x)
+
+
+# ---------------------------------------------------- reference CPU legs --
+def cpu_classify_sample(x, eps_text, m, rows, threads, work_note):
+    """The reference's process_points (data_process.cpp:395-458) over `rows`
+    random chunk rows, extrapolated to its N x N Classify phase."""
+    if not os.path.exists(REF_HARNESS):
+        return None
+    n = len(x)
+    rows = max(16, min(n, int(rows)))
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "in.bin")
+        write_bin(p, x)
+        out = subprocess.run([REF_HARNESS, "classify_sample", p, str(m), eps_text, str(threads), str(rows), "7"],
+                             capture_output=True, text=True, timeout=600, check=True, env=omp_env(threads))
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    full_ms = r["ms"] * (n / r["rows"])
+    return {
+        "value": n / (full_ms * 1e-3), "unit": "points/s", "cores": threads, "kind": "reference",
+        "sample": (f"reference process_points (OpenMP, {threads} threads) over {r['rows']} random chunk rows x "
+                   f"{n} points = {r['pair_tests']:.3g} pair tests in {r['ms'] / 1e3:.1f} s, extrapolated to the "
+                   f"Classify phase's {n} x {n} tests ({full_ms / 1e3:.4g} s); {work_note}"),
+        "measured": "lower bound: Classify phase only, extrapolated from sampled rows",
+        "pair_tests_per_s": r["tests_per_s"],
+    }
+
+
+def cpu_reference_run(x, eps_text, m, threads):
+    """The reference CLI (OpenMP build) on the whole input: its own printed
+    phase times (nextdbscan.cpp:39-65)."""
+    if not os.path.exists(REF_CLI):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "in.bin")
+        write_bin(p, x)
+        out = subprocess.run([REF_CLI, "-m", str(m), "-e", eps_text, "-t", str(threads), p], capture_output=True,
+                             text=True, timeout=900, check=True, env=omp_env(threads)).stdout
+
+    def ms(label):
+        mm = re.search(re.escape(label) + r"\s*(\d+)", out)
+        return int(mm.group(1)) if mm else None
+
+    tri = [int(re.search(r + r":\s*(-?\d+)", out).group(1))
+           for r in ("Core Points", "Noise Points", "(?:Estimated|Upper-bound Approximate) Clusters")]
+    return {"total_ms": ms("Total Execution Time (without I/O):"), "classify_ms": ms("Classify Points:"),
+            "label_ms": ms("Assign Labels:"), "meta": tri}
+
+
+# ------------------------------------------------------------- GPU legs --
 def algorithmic_bytes(kernel, n, d, cells):
     """Bytes one launch must move at minimum (reads of its inputs, writes of
     its outputs), per kernel: N points of d float32, C_ne non-empty cells."""
     table = {
         "count": n * (4 * d + 4) + 8 * cells,          # sorted coords in, nn out, cell table
         "mf_count": n * (4 * d + 4),                    # coordinates in, nn out
-        "union": n * (4 * d + 1) + 8 * cells,          # coords + core flags in, parents
-        "border": n * (4 * d + 1 + 4) + 8 * cells,     # coords + core flags in, labels out
-        "radix_scatter": n * 24,                        # key+val in and out
         "brute_count": n * (4 * d + 4),
     }
     return table.get(kernel)
-
-
-def cpu_baseline(x, eps_text, m, n):
-    """Reference OpenMP Classify on sampled chunk rows (oracle/_ref)."""
-    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
-    if not os.path.exists(harness):
-        return None
-    threads = max(1, min(16, os.cpu_count() or 1))
-    rows = max(64, min(n, int(2.5e10 / max(n, 1))))
-    with tempfile.TemporaryDirectory() as td:
-        p = os.path.join(td, "in.bin")
-        import synth
-        synth.write_bin(p, x)
-        out = subprocess.run([harness, "classify_sample", p, str(m), eps_text, str(threads), str(rows), "7"],
-                             capture_output=True, text=True, timeout=600, check=True,
-                             env=dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores"))
-    r = json.loads(out.stdout.strip().splitlines()[-1])
-    full_ms = r["ms"] * (n / r["rows"])
-    return {
-        "value": n / (full_ms * 1e-3),
-        "unit": "points/s",
-        "cores": threads,
-        "kind": "reference",
-        "sample": (f"reference process_points (data_process.cpp:395-458, OpenMP, {threads} threads) over "
-                   f"{r['rows']} random chunk rows x {n} points = {r['pair_tests']:.3g} pair tests in "
-                   f"{r['ms'] / 1e3:.1f} s, extrapolated to the Classify phase's {n}x{n} tests "
-                   f"({full_ms / 1e3:.4g} s); Assign Labels (>= one more N x N_core sweep) excluded, "
-                   f"so this is an upper bound on the reference's points/s"),
-        "pair_tests_per_s": r["tests_per_s"],
-    }
 
 
 def traffic_from_profiles(kernel, workload):
@@ -95,14 +152,279 @@ def traffic_from_profiles(kernel, workload):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
-def Ctx_stats(ctx):
-    import ndbx
-    return ndbx.Context.stats(ctx)
+def roofline_for(kernel, avg_s, n, d, stats, sum_nn, workload):
+    """Roofline of the dominant kernel from its average launch time."""
+    if kernel == "mf_count":
+        # F = 2d flops (a d-term dot product) per unordered point pair of the
+        # candidate cell pairs = d per ordered pair (pair_tests counts ordered
+        # pairs); issued: 3 bf16 MFMAs per 16-wide k step (hi*hi, hi*lo,
+        # lo*hi), k padded to kp, each unordered pair once
+        kp = 16 if d <= 16 else 32 if d <= 32 else 64 if d <= 64 else 128
+        flops = 1.0 * d * float(stats["pair_tests"])
+        tf = flops / avg_s / 1e12
+        return {"bound": "mfma", "kernel": kernel, "achieved": tf, "peak": BF16_DENSE_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": tf / BF16_DENSE_PEAK_TFLOPS,
+                "traffic": traffic_from_profiles(kernel, workload), "algorithmic_flops": flops,
+                "basis": "2d flops per unordered pair of candidate cells", "avg_ms": avg_s * 1e3,
+                "issued_mfma_tflops_max": tf * 3.0 * kp / d,
+                "issued_note": "upper bound: every block's three bf16 products issued"}
+    ab = algorithmic_bytes(kernel, n, d, stats["cells"])
+    if ab is None:
+        return None
+    gbs = ab / avg_s / 1e9
+    hbm = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+           "algorithmic_bytes": ab}
+    if kernel not in ("count", "brute_count") or sum_nn is None:
+        return {"bound": "hbm", "kernel": kernel, "avg_ms": avg_s * 1e3,
+                "traffic": traffic_from_profiles(kernel, workload), **hbm}
+    # algorithmic arithmetic: d subtractions, d products and d sums per
+    # ordered neighbour pair (self included), 3d * sum(nn)
+    flops = 3.0 * d * float(sum_nn)
+    tf = flops / avg_s / 1e12
+    out = {"bound": "valu", "kernel": kernel, "achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+           "frac": tf / FP32_PEAK_TFLOPS, "traffic": traffic_from_profiles(kernel, workload),
+           "algorithmic_flops": flops, "basis": "3d flops per ordered neighbour pair, 3d * sum(nn)",
+           "sum_nn": int(sum_nn), "avg_ms": avg_s * 1e3,
+           # each flop of a test is its own instruction (no FMA: the reference
+           # rounds every product and sum), so the ceiling is the non-packed
+           # lane-operation rate, half the FMA flop rate
+           "peak_nonpacked_ops": FP32_NONPACKED_PEAK_TFLOPS / 2.0,
+           "frac_nonpacked_ops": tf / (FP32_NONPACKED_PEAK_TFLOPS / 2.0),
+           "hbm": hbm}
+    if stats["pair_tests"]:
+        # issued: the pair tests the kernel ran (parts of dense cells counted
+        # from tables run none), d - 1 sums per test after the first square
+        iflops = (3.0 * d - 1.0) * float(stats["pair_tests"])
+        out["issued"] = {"pair_tests": int(stats["pair_tests"]), "flops": iflops,
+                         "achieved": iflops / avg_s / 1e12,
+                         "frac": iflops / avg_s / 1e12 / FP32_PEAK_TFLOPS,
+                         # each test decides one unordered pair of distinct points
+                         "tests_per_unordered_pair": float(stats["pair_tests"]) / max(1.0, (float(sum_nn) - n) / 2)}
+        if out["frac"] > 1.0:
+            # dense 2-D cells: strip tables count whole rows of neighbours with
+            # no arithmetic, so 3d * sum(nn) is not work the kernel has to do;
+            # the arithmetic it issues is the roof's basis instead
+            out.update({"achieved": out["issued"]["achieved"], "frac": out["issued"]["frac"],
+                        "algorithmic_flops": iflops,
+                        "basis": "(3d - 1) flops per issued pair test: dense-cell tables count most neighbour "
+                                 "pairs without arithmetic, so 3d * sum(nn) exceeds the work done",
+                        "frac_nonpacked_ops": out["issued"]["achieved"] / (FP32_NONPACKED_PEAK_TFLOPS / 2.0)})
+    return out
 
 
-def ctx_path(ctx):
+def measure(ctx, step, steps, warmup, table_steps=3):
+    """Warm up, time `steps` steps with events around the dominant kernel
+    only, then a few more steps with every launch evented (per-kernel table
+    and pair-test statistics)."""
+    import torch
+    for i in range(warmup):
+        if i == warmup - 1:
+            ctx.profile(reset=True)
+        step()
+    warm = ctx.profile(reset=True)
+    dominant = next((k for k in ("mf_count", "count", "brute_count") if k in warm), None)
+    if dominant is None and warm:
+        dominant = max(warm.items(), key=lambda kv: kv[1][0])[0]
+    ctx.profile_select(dominant)
+    torch.cuda.synchronize()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        meta = step()
+    ctx.sync()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    prof = ctx.profile(reset=True)
+    ctx.profile_select(None)
+    for _ in range(table_steps):
+        step()
+    ctx.sync()
+    table = ctx.profile()
+    kernels = {k: {"ms_avg": v[0] / max(v[1], 1), "launches_per_step": v[1] / table_steps}
+               for k, v in sorted(table.items(), key=lambda kv: -kv[1][0])}
+    avg_s = prof[dominant][0] / prof[dominant][1] * 1e-3 if dominant in prof else None
+    return {"ms_per_step": ms, "meta": meta, "dominant": dominant, "avg_s": avg_s, "kernels": kernels,
+            "kernels_pass": f"{table_steps} more steps with every launch evented (the timed steps event only "
+                            f"{dominant})"}
+
+
+def bench_single(workload, steps, warmup, n=None, cpu=True, compare_plain=False, audit=False, graph=False,
+                 threads=16, top_kernels=None):
+    import numpy as np
     import ndbx
-    return ndbx.Context.path(ctx)
+    import synth
+
+    x, eps, m = synth.make(workload, n=n)
+    n, d = x.shape
+    eps_text = EPS_TEXT.get(workload, repr(float(eps)))
+    out = {"workload": workload, "n": n, "d": d, "eps": float(eps), "min_pts": m}
+    with ndbx.Context(x, eps, m, flags=ndbx.PROFILE) as ctx:
+        r = measure(ctx, lambda: ctx.classify().label().meta(), steps, warmup)
+        stats = ctx.stats()
+        nn = ctx.nn()
+        out["path"] = ctx.path()
+    sum_nn = int(nn.astype(np.int64).sum())
+    meta = list(r["meta"])
+    out.update({"ms_per_step": r["ms_per_step"], "value": n / (r["ms_per_step"] * 1e-3), "result": meta,
+                "cells": stats["cells"], "tiles": stats["tiles"], "pair_tests": stats["pair_tests"],
+                "sum_nn": sum_nn})
+    out["roofline"] = roofline_for(r["dominant"], r["avg_s"], n, d, stats, sum_nn, workload) if r["avg_s"] else None
+    kernels = r["kernels"]
+    if top_kernels:
+        kernels = dict(list(kernels.items())[:top_kernels])
+    out["kernels"] = kernels
+    out["kernels_pass"] = r["kernels_pass"]
+    if compare_plain:
+        # plain GPU form: the all-pairs float32 kernels, one timed step
+        import torch
+        with ndbx.Context(x, eps, m, flags=ndbx.FORCE_BRUTE) as pc:
+            pc.classify().label()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            pc.classify().label()
+            pmeta = pc.meta()
+            plain_ms = (time.perf_counter() - t0) * 1e3
+            same = bool(list(pmeta) == meta and np.array_equal(pc.nn(), nn))
+        out["plain_gpu"] = {"ms_per_step": plain_ms, "value": n / (plain_ms * 1e-3), "same_result": same,
+                            "form": "all-pairs float32 kernels (FORCE_BRUTE), 1 timed step",
+                            "speedup": plain_ms / r["ms_per_step"]}
+        assert same, "MFMA filter and all-pairs kernels disagree"
+    if audit:
+        with ndbx.Context(x, eps, m, flags=ndbx.FP64_AUDIT) as ac:
+            ac.classify()
+            a = ac.audit()
+        a["scope"] = "every pair of ~512 sampled 32-row groups of the candidate cell pairs"
+        out["fp64_audit"] = a
+        assert a["filter_flips"] == 0, f"filter flips: {a}"
+    if graph and d <= 3:
+        # the same pass replayed from a captured graph (one-instance batch:
+        # first run eager + capture, then replays)
+        import torch
+        off1 = np.array([0, n], np.int64)
+        with ndbx.Batch(x, off1, np.array([eps], np.float32), np.array([m], np.int32), flags=ndbx.GRAPH) as gb:
+            for _ in range(max(2, warmup)):
+                gb.run()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                gb.run()
+                gmeta = gb.meta()
+            torch.cuda.synchronize()
+            g_ms = (time.perf_counter() - t0) * 1e3 / steps
+            _, replays, reruns = gb.info()
+            same = bool(np.array_equal(gb.nn(), nn) and [int(v) for v in gmeta[0]] == meta)
+        out["graph"] = {"ms_per_step": g_ms, "value": n / (g_ms * 1e-3), "replays": replays, "reruns": reruns,
+                        "same_result": same}
+        assert same, "graph replay differs from the eager run"
+    if cpu:
+        if workload == "blobs2d_100k":
+            t0 = time.perf_counter()
+            ref = cpu_reference_run(x, eps_text, m, threads)
+            if ref:
+                out["cpu_baseline"] = {
+                    "value": n / (ref["total_ms"] * 1e-3), "unit": "points/s", "cores": threads,
+                    "kind": "reference", "measured": "full run",
+                    "sample": (f"reference CLI (OpenMP build, -t {threads}) on all {n} points: Total Execution "
+                               f"Time (without I/O) {ref['total_ms']} ms (Classify {ref['classify_ms']} ms, "
+                               f"Assign Labels {ref['label_ms']} ms); wall {time.perf_counter() - t0:.1f} s"),
+                    "same_result": ref["meta"] == [meta[0], meta[1], meta[2]],
+                }
+        else:
+            # about 1e11 pair tests of CPU work for d = 2, 3 (~10 s on 16
+            # cores); fewer rows for high d, whose tests cost ~d each
+            budget = 1.0e11 if d <= 3 else 3.0e11 / d
+            out["cpu_baseline"] = cpu_classify_sample(
+                x, eps_text, m, budget / n, threads,
+                "Assign Labels (at least one more N x N_core sweep) excluded, so this is an upper bound on the "
+                "reference's points/s")
+    return out
+
+
+def bench_batch(n_inst, steps, warmup, cpu=True, threads=16):
+    import numpy as np
+    import torch
+    import ndbx
+    import synth
+
+    x, off, eps_arr = synth.batch3d(n_inst=n_inst)
+    n, d = x.shape
+    m = 8
+    mins = np.full(n_inst, m, np.int32)
+    out = {"workload": f"batch3d_{n_inst}", "instances": n_inst, "n": n, "d": d, "min_pts": m,
+           "eps_range": [float(eps_arr.min()), float(eps_arr.max())]}
+    with ndbx.Batch(x, off, eps_arr, mins, flags=ndbx.PROFILE) as b:
+        r = measure(b, lambda: b.run().meta(), steps, warmup, table_steps=2)
+        bmeta = b.meta()
+        nn = b.nn()
+        stats = ndbx.Context.stats(b)
+    sum_nn = int(nn.astype(np.int64).sum())
+    out.update({"eager": {"ms_per_step": r["ms_per_step"], "value": n / (r["ms_per_step"] * 1e-3)},
+                "cells": stats["cells"], "tiles": stats["tiles"], "pair_tests": stats["pair_tests"],
+                "sum_nn": sum_nn, "result_sum": [int(v) for v in bmeta.sum(0)]})
+    out["roofline"] = roofline_for(r["dominant"], r["avg_s"], n, d, stats, sum_nn,
+                                   f"batch3d_{n_inst}") if r["avg_s"] else None
+    out["kernels"] = dict(list(r["kernels"].items())[:8])
+    # the batch replayed from a captured graph, tile shape from the autotuner
+    with ndbx.Batch(x, off, eps_arr, mins, flags=ndbx.GRAPH | ndbx.AUTOTUNE) as gb:
+        for _ in range(max(2, warmup)):
+            gb.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            gb.run()
+            gmeta = gb.meta()
+        torch.cuda.synchronize()
+        g_ms = (time.perf_counter() - t0) * 1e3 / steps
+        gc, replays, reruns = gb.info()
+        same = bool(np.array_equal(gmeta, bmeta) and np.array_equal(gb.nn(), nn))
+    out["graph"] = {"ms_per_step": g_ms, "value": n / (g_ms * 1e-3), "replays": replays, "reruns": reruns,
+                    "group_cells": gc, "same_result": same, "form": "graph replay, autotuned tile shape"}
+    assert same, "graph replay differs from the eager batched run"
+    # plain form: one context per instance, device-resident points
+    x_dev = torch.from_numpy(x).to("cuda")
+    plain = [ndbx.Context(None, eps_arr[i], m, device_ptr=x_dev.data_ptr() + int(off[i]) * 4 * d,
+                          n=int(off[i + 1] - off[i]), d=d) for i in range(n_inst)]
+    for c in plain:  # warm: every context's buffers allocated
+        c.classify().label()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pmeta = [c.classify().label().meta() for c in plain]
+    torch.cuda.synchronize()
+    plain_ms = (time.perf_counter() - t0) * 1e3
+    for c in plain:
+        c.close()
+    del x_dev
+    mism = [i for i in range(n_inst) if tuple(bmeta[i]) != tuple(pmeta[i])]
+    out["plain_gpu"] = {"ms_per_step": plain_ms, "value": n / (plain_ms * 1e-3),
+                        "form": "one context per instance, run one after another, 1 timed step",
+                        "same_result": not mism, "instances_compared": n_inst}
+    assert not mism, f"batched and per-instance runs differ on instances {mism[:10]}"
+    if cpu and os.path.exists(REF_CLI):
+        # the reference CLI on whole instances, smallest-first over a spread
+        # of sizes until ~15 s of CPU work; extrapolated by sum(n^2), the
+        # quadratic cost of both of its phases
+        sizes = np.diff(off).astype(np.float64)
+        order = np.argsort(sizes)
+        pick = order[np.linspace(0, n_inst - 1, 48).round().astype(np.int64)]
+        done, tot_ms, sq = [], 0.0, 0.0
+        t0 = time.perf_counter()
+        for i in pick:
+            if time.perf_counter() - t0 > 15.0:
+                break
+            s, e = int(off[i]), int(off[i + 1])
+            ref = cpu_reference_run(x[s:e], repr(float(eps_arr[i])), m, threads)
+            tot_ms += ref["total_ms"]
+            sq += sizes[i] ** 2
+            done.append(int(i))
+        full_ms = tot_ms * float((sizes ** 2).sum()) / sq
+        out["cpu_baseline"] = {
+            "value": n / (full_ms * 1e-3), "unit": "points/s", "cores": threads, "kind": "reference",
+            "measured": "sampled instances, extrapolated by sum(n^2)",
+            "sample": (f"reference CLI (OpenMP build, -t {threads}) on {len(done)} whole instances (sizes "
+                       f"{int(sizes[done].min())}-{int(sizes[done].max())}), sum of Total Execution Time (without "
+                       f"I/O) {tot_ms / 1e3:.2f} s, scaled by sum(n^2) of all instances / of the sample = "
+                       f"{float((sizes ** 2).sum()) / sq:.1f} to {full_ms / 1e3:.4g} s")}
+    return out
 
 
 def main():
@@ -111,15 +433,12 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD)
-    ap.add_argument("--n", type=int, default=None, help="smaller instance of the same workload shape")
+    ap.add_argument("--n", type=int, default=None, help="smaller instance of the headline workload's shape")
     ap.add_argument("--n-inst", type=int, default=4096, help="instances of the batched workload")
+    ap.add_argument("--headline-only", action="store_true", help="skip the other workloads' forms")
+    ap.add_argument("--forms", default="blobs2d_100k,clusters64d_1m,batch3d,skew2d_100m",
+                    help="comma-separated list of the other workloads to measure")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU form comparison")
-    ap.add_argument("--no-audit", action="store_true", help="skip the float64 audit of the MFMA filter")
-    ap.add_argument("--graph", action="store_true",
-                    help="also time graph replays (NDBX_GRAPH; 3-D and below) and report them as value")
-    ap.add_argument("--autotune", action="store_true",
-                    help="batched workload: tile shape from the autotuner (NDBX_AUTOTUNE)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one device-to-device copy of BYTES (a known-size kernel for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -129,10 +448,8 @@ def main():
               "(RCCL/xGMI slab sharding) path is a design note, not built.", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-    import torch
+    import torch  # before the library: both then use the one HIP runtime loaded first
     import ndbx
-    import synth
 
     if not ndbx.device_available():
         raise RuntimeError("no HIP device")
@@ -142,205 +459,45 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    host = host_cpu()
+    threads = physical_cores(host)
+    cpu = not args.no_cpu_baseline
+    t_start = time.perf_counter()
 
-    extra = {}
+    def progress(msg):
+        print(f"[bench {time.perf_counter() - t_start:6.1f}s] {msg}", file=sys.stderr, flush=True)
+
+    progress(f"headline {args.workload}")
     if args.workload.startswith("batch3d"):
-        # many independent instances: one batched run vs the plain form (each
-        # instance its own context), both from device-resident points
-        n_inst = args.n_inst
-        x, off, eps_arr = synth.batch3d(n_inst=n_inst)
-        n, d = x.shape
-        m = 8
-        mins = np.full(n_inst, m, np.int32)
-        tune = ndbx.AUTOTUNE if args.autotune else 0
-        ctx = ndbx.Batch(x, off, eps_arr, mins, flags=ndbx.PROFILE | tune)
-        step = lambda: (ctx.run(), ctx.meta())
-        eps = float(eps_arr.mean())
-        eps_text = None
-        x_dev = torch.from_numpy(x).to("cuda")
-        n_plain = 0 if args.no_plain else n_inst
-        plain = [ndbx.Context(None, eps_arr[i], m, device_ptr=x_dev.data_ptr() + int(off[i]) * 4 * d,
-                              n=int(off[i + 1] - off[i]), d=d) for i in range(n_plain)]
-        for c in plain:  # warm: allocate every context's buffers
-            c.classify().label()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(max(1, args.steps // 5)):
-            for c in plain:
-                c.classify().label()
-                c.meta()
-        torch.cuda.synchronize()
-        plain_ms = (time.perf_counter() - t0) * 1e3 / max(1, args.steps // 5)
-        # the plain form must give the batched form's per-instance results
-        ctx.run()
-        bmeta = ctx.meta()
-        for i in range(0, n_plain, max(1, n_inst // 64)):
-            assert tuple(bmeta[i]) == plain[i].meta(), f"instance {i} differs"
-        for c in plain:
-            c.close()
-        if n_plain:
-            extra["plain_gpu"] = {"ms_per_step": plain_ms, "value": n / (plain_ms * 1e-3),
-                                  "form": "one context per instance, run one after another"}
-        extra["instances"] = n_inst
-        if args.graph:
-            # the same batch replayed from a captured graph (first run eager + capture)
-            with ndbx.Batch(x, off, eps_arr, mins, flags=ndbx.GRAPH | tune) as gb:
-                for _ in range(max(1, args.warmup)):
-                    gb.run()
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                for _ in range(args.steps):
-                    gb.run()
-                    gmeta = gb.meta()
-                torch.cuda.synchronize()
-                g_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-                gc, replays, reruns = gb.info()
-                assert np.array_equal(gmeta, bmeta), "graph replay differs from the eager batched run"
-            extra["graph"] = {"ms_per_step": g_ms, "value": n / (g_ms * 1e-3), "replays": replays,
-                              "reruns": reruns, "group_cells": gc}
+        h = bench_batch(args.n_inst, args.steps, args.warmup, cpu=cpu, threads=threads)
+        ms_per_step = h["eager"]["ms_per_step"]
     else:
-        x, eps, m = synth.make(args.workload, n=args.n)
-        n, d = x.shape
-        eps_text = {"blobs2d_100k": "0.5", "blobs3d_10m": "1.5", "skew2d_100m": "0.3"}.get(args.workload,
-                                                                                             repr(float(eps)))
-        ctx = ndbx.Context(x, eps, m, flags=ndbx.PROFILE)
-        step = lambda: (ctx.classify().label(), ctx.meta())
-        if d > 3 and not args.no_plain:
-            # plain GPU form for comparison: the all-pairs float32 kernels
-            with ndbx.Context(x, eps, m, flags=ndbx.FORCE_BRUTE) as pc:
-                pc.classify().label()
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                pc.classify().label()
-                pmeta = pc.meta()
-                plain_ms = (time.perf_counter() - t0) * 1e3
-                plain_nn = pc.nn()
-            extra["plain_gpu"] = {"ms_per_step": plain_ms, "value": n / (plain_ms * 1e-3),
-                                  "form": "all-pairs float32 kernels (FORCE_BRUTE), 1 timed step"}
-        if args.graph and d <= 3:
-            # the same pass replayed from a captured graph (one-instance batch:
-            # first run eager + capture, then replays)
-            off1 = np.array([0, n], np.int64)
-            with ndbx.Batch(x, off1, np.array([eps], np.float32), np.array([m], np.int32), flags=ndbx.GRAPH) as gb:
-                for _ in range(max(1, args.warmup)):
-                    gb.run()
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                for _ in range(args.steps):
-                    gb.run()
-                    gmeta = gb.meta()
-                torch.cuda.synchronize()
-                g_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-                gc, replays, reruns = gb.info()
-                gnn = gb.nn()
-            extra["graph"] = {"ms_per_step": g_ms, "value": n / (g_ms * 1e-3), "replays": replays,
-                              "reruns": reruns, "group_cells": gc, "meta": [int(v) for v in gmeta[0]]}
-            extra["graph_nn"] = gnn
-        if d > 3 and not args.no_audit:
-            with ndbx.Context(x, eps, m, flags=ndbx.FP64_AUDIT) as ac:
-                ac.classify()
-                extra["fp64_audit"] = ac.audit()
-                extra["fp64_audit"]["scope"] = "every pair of ~2048 sampled 32-row tiles"
-
-    for i in range(args.warmup):
-        if i == args.warmup - 1:
-            ctx.profile(reset=True)
-        step()
-    # The timed steps keep HIP events around the dominant kernel only (the
-    # counting kernel, else the last warmup step's longest): events around
-    # every launch add host work per launch that shows up in the step time.
-    # Without warmup every launch is evented.
-    warm = ctx.profile(reset=True)
-    dominant = next((k for k in ("mf_count", "count") if k in warm), None)
-    if dominant is None and warm:
-        dominant = max(warm.items(), key=lambda kv: kv[1][0])[0]
-    ctx.profile_select(dominant)
-    torch.cuda.synchronize()
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        _, meta = step()
-    ctx.sync()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    prof = ctx.profile(reset=True)
-    if dominant is None and prof:
-        dominant = next((k for k in ("mf_count", "count") if k in prof), None) or \
-            max(prof.items(), key=lambda kv: kv[1][0])[0]
-    # per-kernel table: a separate pass of the same step with every launch evented
-    ctx.profile_select(None)
-    table_steps = max(1, min(args.steps, 3))
-    for _ in range(table_steps):
-        step()
-    ctx.sync()
-    table = ctx.profile()
-    stats = Ctx_stats(ctx)
-    path = ctx_path(ctx)
-    nn = ctx.nn()
-    if isinstance(meta, np.ndarray):
-        meta = [int(v) for v in meta.sum(0)]
-    ctx.close()
-    if "plain_gpu" in extra and not args.workload.startswith("batch3d"):
-        extra["plain_gpu"]["same_result"] = bool(tuple(pmeta) == tuple(meta) and np.array_equal(plain_nn, nn))
-
-    kernels = {k: {"ms_avg": v[0] / max(v[1], 1), "launches_per_step": v[1] / table_steps}
-               for k, v in sorted(table.items(), key=lambda kv: -kv[1][0])}
-    roofline = None
-    if dominant and dominant in prof:
-        total_ms, launches = prof[dominant]
-        avg_s = total_ms / launches * 1e-3
-        ab = algorithmic_bytes(dominant, n, d, stats["cells"])
-        if dominant == "mf_count":
-            # distance filter: F = 2d flops (a d-term dot product) per unordered
-            # point pair of the candidate cell pairs, i.e. d per ordered pair
-            # (pair_tests counts ordered pairs); the kernel computes each
-            # unordered pair once with 3 bf16 MFMAs per 16-wide k step (hi*hi,
-            # hi*lo, lo*hi), k padded to kp: 3 kp issued flops per ordered pair
-            kp = 16 if d <= 16 else 32 if d <= 32 else 64 if d <= 64 else 128
-            flops = 1.0 * d * float(stats["pair_tests"])
-            tf = flops / avg_s / 1e12
-            roofline = {"bound": "mfma", "kernel": dominant, "achieved": tf, "peak": BF16_DENSE_PEAK_TFLOPS,
-                        "unit": "TFLOP/s", "frac": tf / BF16_DENSE_PEAK_TFLOPS,
-                        "traffic": traffic_from_profiles(dominant, args.workload),
-                        "algorithmic_flops": flops, "avg_ms": avg_s * 1e3,
-                        "issued_mfma_tflops": tf * 3.0 * kp / d,
-                        "issued_frac": tf * 3.0 * kp / d / BF16_DENSE_PEAK_TFLOPS}
-        elif ab is not None:
-            achieved = ab / avg_s / 1e9
-            roofline = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                        "traffic": traffic_from_profiles(dominant, args.workload),
-                        "algorithmic_bytes": ab, "avg_ms": avg_s * 1e3}
-            if dominant in ("count", "brute_count") and stats["pair_tests"]:
-                # compute view: the arithmetic issued, d differences, d products
-                # and d - 1 sums per pair test (pairs inside dense cells'
-                # strip rows are counted from tables, without tests)
-                flops = (3.0 * d - 1.0) * float(stats["pair_tests"])
-                tf = flops / avg_s / 1e12
-                roofline["valu"] = {"achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                                    "frac": tf / FP32_PEAK_TFLOPS, "flops": flops,
-                                    "basis": "issued pair tests x (3d - 1) flops",
-                                    # each flop of a test is its own instruction (no FMA: the
-                                    # reference rounds every product and sum), so the ceiling is
-                                    # the non-packed lane-operation rate, half the FMA flop rate
-                                    "peak_nonpacked_ops": FP32_NONPACKED_PEAK_TFLOPS / 2.0,
-                                    "frac_nonpacked_ops": tf / (FP32_NONPACKED_PEAK_TFLOPS / 2.0)}
-    cpu = None if (args.no_cpu_baseline or eps_text is None) else cpu_baseline(x, eps_text, m, n)
-
-    gnn = extra.pop("graph_nn", None)
-    if gnn is not None:
-        # the replays must give the eager pass's results
-        assert np.array_equal(gnn, nn) and extra["graph"]["meta"] == list(meta), "graph replay differs"
-    config_extra = {}
-    if "graph" in extra:
-        # value: graph replays; the profiled eager batched run gives the kernel table
-        config_extra = {"form": "graph replay", "eager_ms_per_step": ms_per_step,
-                        "group_cells": extra["graph"]["group_cells"]}
-        ms_per_step = extra["graph"]["ms_per_step"]
+        h = bench_single(args.workload, args.steps, args.warmup, n=args.n, cpu=cpu, threads=threads,
+                         compare_plain=args.workload == "clusters64d_1m",
+                         audit=args.workload == "clusters64d_1m")
+        ms_per_step = h["ms_per_step"]
+    forms = {}
+    if not args.headline_only:
+        for f in [s for s in args.forms.split(",") if s]:
+            if f == args.workload or (f == "batch3d" and args.workload.startswith("batch3d")):
+                continue
+            progress(f"form {f}")
+            if f == "blobs2d_100k":
+                forms[f] = bench_single(f, 20, 3, cpu=cpu, graph=True, threads=threads, top_kernels=6)
+            elif f == "clusters64d_1m":
+                forms[f] = bench_single(f, 5, 2, cpu=cpu, compare_plain=True, audit=True, threads=threads,
+                                        top_kernels=6)
+            elif f == "batch3d":
+                forms[f"batch3d_{args.n_inst}"] = bench_batch(args.n_inst, 5, 2, cpu=cpu, threads=threads)
+            elif f == "skew2d_100m":
+                forms[f] = bench_single(f, 3, 1, cpu=cpu, threads=threads, top_kernels=6)
+            torch.cuda.empty_cache()
+    progress("done")
+    config = {k: h[k] for k in ("workload", "n", "d", "eps", "min_pts", "path", "cells", "tiles", "pair_tests",
+                                "sum_nn", "result", "instances") if k in h}
     out = {
         "metric": METRIC,
-        "value": n / (ms_per_step * 1e-3),
+        "value": h["n"] / (ms_per_step * 1e-3),
         "unit": "points/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -351,14 +508,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded numpy generator, nextdbscan-exa_amd/synth.py)",
-        "config": {"workload": args.workload, "n": n, "d": d, "eps": float(eps), "min_pts": m, "path": path,
-                   "cells": stats["cells"], "tiles": stats["tiles"], "pair_tests": stats["pair_tests"], "result": list(meta), **config_extra},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "kernels": kernels,
-        "kernels_pass": f"{table_steps} more steps with every launch evented (the timed steps event only {dominant})",
+        "config": config,
+        "roofline": h.get("roofline"),
+        "cpu_baseline": h.get("cpu_baseline"),
+        "host": host,
+        "kernels": h.get("kernels"),
+        "kernels_pass": h.get("kernels_pass"),
+        "forms": forms,
+        "bench_wall_s": time.perf_counter() - t_start,
     }
-    out.update(extra)
+    for k in ("plain_gpu", "fp64_audit", "graph", "eager"):
+        if k in h:
+            out[k] = h[k]
     print(json.dumps(out), flush=True)
 
 
