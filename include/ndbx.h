@@ -37,9 +37,11 @@ typedef enum ndbx_status {
 
 /* Flags for ndbx_create / ndbx_batch_create. */
 enum {
-    /* -a 1 (main.cpp:45, data_process.cpp:215).  Cores and noise are the same
-     * as in exact mode; this build then returns the exact labelling, which is
-     * the tightest of the reference's upper-bound cluster counts. */
+    /* -a 1 (main.cpp:45, data_process.cpp:214-219): the reference stops after
+     * one label sweep, so its cluster count is an upper bound while cores and
+     * noise are the same as in exact mode.  This build returns the exact
+     * labelling in both modes: the tightest of those upper bounds.  Flag bits
+     * not defined here are refused (NDBX_EINVAL). */
     NDBX_APPROX = 1u << 0,
     /* Force the all-pairs kernels even where the cell grid applies. */
     NDBX_FORCE_BRUTE = 1u << 1,

@@ -53,6 +53,13 @@ static void need(bool ok, const char *msg) {
     if (!ok) throw Error(NDBX_EINVAL, msg);
 }
 
+// Every flag bit include/ndbx.h defines.  NDBX_APPROX is accepted and runs
+// the exact labelling: cores and noise are the same in both of the
+// reference's modes (main.cpp:45), and the exact cluster count is the
+// tightest of its upper-bound approximate counts (data_process.cpp:214-219).
+constexpr uint32_t kKnownFlags = NDBX_APPROX | NDBX_FORCE_BRUTE | NDBX_DEVICE_INPUT | NDBX_PROFILE |
+                                 NDBX_MFMA_FILTER | NDBX_FP64_AUDIT | NDBX_GRAPH | NDBX_AUTOTUNE;
+
 static void open_common(Ctx &c, const float *coords, int64_t n_total, uint32_t flags) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw Error(NDBX_EDEVICE, "no HIP device");
@@ -286,6 +293,7 @@ int ndbx_create(ndbx_ctx **out, const float *coords, int64_t n, int d, float eps
         need(n < (int64_t)INT32_MAX - 64, "n must be below 2^31 - 64 (the reference's .bin header is int32)");
         need(d >= 1, "d < 1");
         need(n == 0 || coords != nullptr, "coords is null");
+        need((flags & ~kKnownFlags) == 0, "unknown flag bits");
         auto *h = new ndbx_ctx();
         Ctx &c = h->c;
         c.d = d;
@@ -316,6 +324,7 @@ int ndbx_batch_create(ndbx_ctx **out, const float *coords, const int64_t *offset
         const int64_t n = offsets[n_inst];
         need(n < (int64_t)INT32_MAX - 64, "total points must be below 2^31 - 64");
         need(n == 0 || coords != nullptr, "coords is null");
+        need((flags & ~kKnownFlags) == 0, "unknown flag bits");
         auto *h = new ndbx_ctx();
         Ctx &c = h->c;
         c.d = d;

@@ -24,6 +24,7 @@ data_process::data_process(h_vec<float> &v_coord, int const m, float const e, in
     uint32_t flags = is_approximate ? NDBX_APPROX : 0u;
     if (std::getenv("NDBX_FORCE_BRUTE")) flags |= NDBX_FORCE_BRUTE;
     check(ndbx_create(&ctx, v_coord.data(), (int64_t)n_coord, n_dim, e, m, flags), "ndbx_create");
+    this->v_coord = std::move(v_coord);
     allocated_bytes = (unsigned long long)n_coord * (unsigned long long)n_dim * sizeof(float);
 }
 
@@ -31,6 +32,8 @@ data_process::~data_process() { ndbx_destroy(ctx); }
 
 void data_process::select_and_process(magmaMPI) noexcept {
     check(ndbx_classify(ctx), "ndbx_classify");
+    v_coord_id.resize(n_coord);
+    for (std::size_t i = 0; i < n_coord; ++i) v_coord_id[i] = (int)i;
     v_coord_nn.resize(n_coord);
     check(ndbx_get_nn(ctx, v_coord_nn.data()), "ndbx_get_nn");
 }
@@ -39,6 +42,11 @@ void data_process::select_cores_and_process(magmaMPI) noexcept {
     check(ndbx_label(ctx), "ndbx_label");
     v_coord_cluster_index.resize(n_coord);
     check(ndbx_get_labels(ctx, v_coord_cluster_index.data()), "ndbx_get_labels");
+    // label table over core ranks: the cluster index of each core, in
+    // ascending point id (every entry is already its root)
+    v_cluster_label.clear();
+    for (std::size_t i = 0; i < n_coord; ++i)
+        if (v_coord_nn[i] >= m) v_cluster_label.push_back(v_coord_cluster_index[i]);
 }
 
 void data_process::get_result_meta(int &cores, int &noise, int &clusters, int &n, magmaMPI) noexcept {

@@ -32,8 +32,19 @@ class data_process {
 
    public:
     unsigned long long allocated_bytes = 0;
+    // data_process.h:54-59.  v_coord holds the coordinates (moved in, as the
+    // reference's OpenMP build does; the device copy is the context's).
+    // v_coord_id is the point ids 0..n-1 after select_and_process (one rank).
+    // v_cluster_label is the label table over core ranks after
+    // select_cores_and_process: entry k is the cluster of the k-th core in
+    // ascending point id, and a cluster's id is the core rank of its lowest
+    // core, so the roots (entry k == k) are the clusters, as
+    // get_result_meta counts them (data_process.cpp:79-88).
+    h_vec<float> v_coord;
+    h_vec<int> v_coord_id;
     h_vec<int> v_coord_nn;             // per point, input order
     h_vec<int> v_coord_cluster_index;  // per point, input order
+    h_vec<int> v_cluster_label;        // per core rank
 
     // data_process.h:61-76: coordinates are copied to HBM here.
     data_process(h_vec<float> &v_coord, int m, float e, int n_dim, int n_total_coord, bool is_approximate);

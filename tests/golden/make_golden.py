@@ -7,7 +7,7 @@ v_coord_cluster_index, plus its (cores, noise, clusters) triple.  CLI triples
 come from the reference CLI built the same way (oracle/_ref/nextdbscan-omp).
 
 Run in a container that has the reference sources and `make -C oracle`:
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [--approx-only]
 """
 from __future__ import annotations
 
@@ -140,8 +140,45 @@ def ref_cli_triple(x, eps_text, m):
     return [cl, co, no]
 
 
+APPROX_CASES = ["blobs2d_3k", "blobs3d_4k", "lattice2d_exact_eps", "lattice3d_exact_eps", "duplicates2d",
+                "blobs_d5", "skew2d_20k", "all_noise"]
+
+
+def ref_cli_approx(x, eps_text, m):
+    """The reference CLI with -a 1: its printed (clusters, cores, noise)."""
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "in.bin")
+        synth.write_bin(p, x)
+        txt = subprocess.check_output([O.REF_CLI, "-m", str(m), "-e", eps_text, "-t", "8", "-a", "1", p],
+                                      text=True)
+    cl = int(re.search(r"Upper-bound Approximate Clusters: (-?\d+)", txt).group(1))
+    co = int(re.search(r"Core Points: (-?\d+)", txt).group(1))
+    no = int(re.search(r"Noise Points: (-?\d+)", txt).group(1))
+    return [cl, co, no]
+
+
+def write_approx():
+    """approx_triples.json: for each case, the reference data_process run with
+    approx = true ((cores, noise, clusters)) and the reference CLI's -a 1
+    printed (clusters, cores, noise).  The approximate cluster count depends on
+    the sweep's thread schedule; both are one run's values (8 threads)."""
+    out = {}
+    for name, x, eps_text, m in cases():
+        if name not in APPROX_CASES:
+            continue
+        x = np.ascontiguousarray(x, np.float32)
+        _nn, _cci, _lab, meta = O.ref_full(x, eps_text, m, threads=8, approx=True)
+        out[name] = {"data_process_meta": [int(v) for v in meta], "cli": ref_cli_approx(x, eps_text, m)}
+        print(f"{name:24s} approx meta={meta} cli={out[name]['cli']}", flush=True)
+    with open(os.path.join(HERE, "approx_triples.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
 def main():
     assert O.ref_available(), "build oracle/_ref first: make -C oracle"
+    if sys.argv[1:] == ["--approx-only"]:
+        write_approx()
+        return
     triples = {}
     for name, x, eps_text, m in cases():
         x = np.ascontiguousarray(x, np.float32)
@@ -156,6 +193,7 @@ def main():
               f"meta={meta} cli={triples[name]}", flush=True)
     with open(os.path.join(HERE, "cli_triples.json"), "w") as f:
         json.dump(triples, f, indent=1, sort_keys=True)
+    write_approx()
 
 
 if __name__ == "__main__":

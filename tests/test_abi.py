@@ -39,6 +39,8 @@ def test_argument_errors_are_status_codes():
     assert L.ndbx_create(ctypes.byref(h), None, -1, 2, 0.5, 5, 0) == ndbx.EINVAL
     assert L.ndbx_create(ctypes.byref(h), None, 10, 0, 0.5, 5, 0) == ndbx.EINVAL
     assert L.ndbx_create(ctypes.byref(h), None, 10, 2, 0.5, 5, 0) == ndbx.EINVAL  # null coords
+    assert L.ndbx_create(ctypes.byref(h), None, 0, 2, 0.5, 5, 1 << 20) == ndbx.EINVAL  # unknown flag bit
+    assert b"flag" in L.ndbx_last_error()
     assert L.ndbx_classify(None) == ndbx.EINVAL
     assert L.ndbx_label(None) == ndbx.EINVAL
     assert L.ndbx_destroy(None) == ndbx.OK  # like free(NULL)

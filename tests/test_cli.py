@@ -70,3 +70,83 @@ def test_cli_csv_input(gpu_lib, tmp_path):
     assert "WARNING: USING SLOW CSV I/O." in r.stdout
     co = int(re.search(r"Core Points: (-?\d+)", r.stdout).group(1))
     assert co == TRIPLES["blobs2d_3k"][1]
+
+
+APPROX = json.load(open(os.path.join(GOLDEN_DIR, "approx_triples.json")))
+
+
+@pytest.mark.parametrize("name", sorted(APPROX))
+def test_reference_approx_contract_holds_on_fixtures(name):
+    """The reference's own -a 1 runs (data_process with approx and the CLI):
+    cores and noise equal the exact run's, the cluster count is at least the
+    exact one (main.cpp:45)."""
+    g = load_golden(name)
+    cores, noise, clusters = APPROX[name]["data_process_meta"]
+    assert (cores, noise) == g["meta"][:2]
+    assert clusters >= g["meta"][2]
+    cl, co, no = APPROX[name]["cli"]
+    assert [co, no] == TRIPLES[name][1:] and cl >= TRIPLES[name][0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(APPROX))
+def test_cli_approx_mode(gpu_lib, tmp_path, name):
+    """-a 1 prints the reference's approximate line; cores and noise equal the
+    reference's -a 1 run, and the count printed (the exact one) is at most
+    the reference's upper bound."""
+    g = load_golden(name)
+    p = str(tmp_path / "in.bin")
+    synth.write_bin(p, g["x"])
+    r = run_cli(["-m", str(g["m"]), "-e", g["eps_text"], "-t", "4", "-a", "1", p])
+    assert r.returncode == 0, r.stderr
+    assert "Estimated Clusters:" not in r.stdout
+    cl = int(re.search(r"Upper-bound Approximate Clusters: (-?\d+)", r.stdout).group(1))
+    co = int(re.search(r"Core Points: (-?\d+)", r.stdout).group(1))
+    no = int(re.search(r"Noise Points: (-?\d+)", r.stdout).group(1))
+    ref_cl, ref_co, ref_no = APPROX[name]["cli"]
+    assert (co, no) == (ref_co, ref_no)
+    assert cl == TRIPLES[name][0] and cl <= ref_cl
+
+
+@pytest.mark.gpu
+def test_approx_flag_same_labels(gpu_lib):
+    g = load_golden("blobs3d_4k")
+    eps = synth.eps_f32(g["eps_text"])
+    with gpu_lib.Context(g["x"], eps, g["m"], flags=gpu_lib.APPROX) as c:
+        c.classify().label()
+        np.testing.assert_array_equal(c.nn(), g["nn"])
+        np.testing.assert_array_equal(c.labels(), g["cci"])
+        assert c.meta() == g["meta"]
+
+
+MEMBERS = os.path.join(ROOT, "tests", "host", "members_check")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["blobs2d_3k", "blobs3d_4k", "duplicates2d", "all_noise", "flt_max_padding"])
+def test_host_data_process_members(gpu_lib, tmp_path, name):
+    """The host data_process's public members after the reference's call
+    sequence: v_coord kept, v_coord_id = 0..n-1, v_coord_nn and
+    v_coord_cluster_index equal to the reference's, and v_cluster_label a
+    label table over core ranks whose roots are the clusters
+    (data_process.cpp:79-88)."""
+    g = load_golden(name)
+    p, out = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    synth.write_bin(p, g["x"])
+    r = subprocess.run([MEMBERS, p, str(g["m"]), g["eps_text"], out], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, np.int32)
+    n, d, n_coord_vals, cores, noise, clusters, n_lab, roots = (int(v) for v in raw[:8])
+    o = 8
+    ids, nn, cci = raw[o:o + n], raw[o + n:o + 2 * n], raw[o + 2 * n:o + 3 * n]
+    lab = raw[o + 3 * n:o + 3 * n + n_lab]
+    coords = raw[o + 3 * n + n_lab:].view(np.float32).reshape(n, d)
+    assert n_coord_vals == n * d
+    np.testing.assert_array_equal(coords, g["x"])
+    np.testing.assert_array_equal(ids, np.arange(n))
+    np.testing.assert_array_equal(nn, g["nn"])
+    np.testing.assert_array_equal(cci, g["cci"])
+    core = g["nn"] >= g["m"]
+    assert n_lab == core.sum() == cores
+    np.testing.assert_array_equal(lab, g["cci"][core])
+    assert roots == clusters == g["meta"][2] and (cores, noise) == g["meta"][:2]
