@@ -439,6 +439,8 @@ def main():
     ap.add_argument("--forms", default="blobs2d_100k,clusters64d_1m,batch3d,skew2d_100m",
                     help="comma-separated list of the other workloads to measure")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the all-pairs comparison of the MFMA workload")
+    ap.add_argument("--no-audit", action="store_true", help="skip the float64 audit of the MFMA filter")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one device-to-device copy of BYTES (a known-size kernel for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -473,8 +475,8 @@ def main():
         ms_per_step = h["eager"]["ms_per_step"]
     else:
         h = bench_single(args.workload, args.steps, args.warmup, n=args.n, cpu=cpu, threads=threads,
-                         compare_plain=args.workload == "clusters64d_1m",
-                         audit=args.workload == "clusters64d_1m")
+                         compare_plain=args.workload == "clusters64d_1m" and not args.no_plain,
+                         audit=args.workload == "clusters64d_1m" and not args.no_audit)
         ms_per_step = h["ms_per_step"]
     forms = {}
     if not args.headline_only:

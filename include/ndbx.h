@@ -138,6 +138,23 @@ int ndbx_profile_select(ndbx_ctx *ctx, const char *name);
  * launch; otherwise the last collected value, 0 if none). */
 int ndbx_stats(ndbx_ctx *ctx, int64_t *cells, int64_t *tiles, int64_t *pair_tests);
 
+/*
+ * Chunked runs (data_process.cpp:285-302, :335-362 bound the reference's
+ * working copies to chunks of 100M points).  With max_slab_points > 0 the
+ * next ndbx_classify / ndbx_label run slab by slab: points sorted by their
+ * first coordinate, each slab a range of that order plus every point within
+ * eps of it in that coordinate, at most max_slab_points points; the grid
+ * pipeline's buffers are sized for one slab, and per-point state (counts,
+ * core flags, labels, core ranks: 17 bytes per point, plus the coordinates)
+ * stays resident.  Results are identical to the resident run.  Takes one
+ * instance on the cell grid (d <= 3, finite coordinates; else NDBX_EUNSUPPORTED
+ * from ndbx_classify), and a budget that holds the points within eps (in the
+ * first coordinate) of every point.  0 restores resident runs.
+ */
+int ndbx_set_chunking(ndbx_ctx *ctx, int64_t max_slab_points);
+/* Slabs of the last chunked classify (0 for resident runs). */
+int ndbx_chunk_info(ndbx_ctx *ctx, int64_t *slabs);
+
 /* MFMA filter audit (NDBX_FP64_AUDIT): guard-band pairs rechecked, decisions
  * that differ between the FP32 predicate and a float64 distance, decisions
  * of the BF16 estimate outside the band that the FP32 predicate contradicts

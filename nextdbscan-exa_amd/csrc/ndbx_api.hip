@@ -26,7 +26,9 @@ Ctx::~Ctx() {
                       &nranges, &ranges, &tile_off, &tiles, &nn_s, &core_s, &cell_core_min, &cell_has_core,
                       &parent, &comp_min, &cell_label, &label_s, &rank, &misc, &scratch, &scratch2, &b_parent,
                       &b_rootmin, &soa_core, &cell_c, &gflag, &ctile_off, &ctiles, &tile_tests, &dense_id, &dense_info, &dense_tmp, &fine_off, &dcount, &plan_in, &part_off,
-                      &pflag, &bq, &cgroups, &part_fine, &big_id, &core_bits, &border_tiles};
+                      &pflag, &bq, &cgroups, &part_fine, &big_id, &core_bits, &border_tiles,
+                      &ch_perm, &ch_grank, &ch_parent, &ch_gid, &ch_coords, &ch_ext, &ch_inv};
+    delete sub;
     for (auto *b : bufs) b->release();
     if (stream && own_stream) (void)hipStreamDestroy(stream);
 }
@@ -89,6 +91,12 @@ static void open_common(Ctx &c, const float *coords, int64_t n_total, uint32_t f
 static void run_classify(Ctx &c) {
     c.labelled = false;
     c.meta_valid = false;
+    if (c.chunk_points > 0 && c.n > 0) {
+        chunked_classify(c);
+        c.classified = true;
+        return;
+    }
+    c.n_slabs = 0;
     grid_plan(c);
     if (c.n == 0) {
         c.classified = true;
@@ -104,6 +112,8 @@ static void run_label(Ctx &c) {
     if (!c.classified) throw Error(NDBX_ESTATE, "ndbx_label before ndbx_classify");
     if (c.n == 0) {
         fill_i32(c.meta.as<int32_t>(), 0, 3 * c.n_inst, c.stream, c.prof);
+    } else if (c.chunk_points > 0) {
+        chunked_label(c);
     } else if (c.path == PATH_GRID) {
         grid_label(c);
     } else if (c.path == PATH_MFMA) {
@@ -528,6 +538,25 @@ int ndbx_audit(ndbx_ctx *h, int64_t *band_pairs, int64_t *fp64_flips, int64_t *f
 
 int ndbx_destroy(ndbx_ctx *h) {
     return guarded([&] { delete h; });
+}
+
+int ndbx_set_chunking(ndbx_ctx *h, int64_t max_slab_points) {
+    return guarded([&] {
+        need(h != nullptr, "ctx is null");
+        need(max_slab_points >= 0, "max_slab_points < 0");
+        Ctx &c = h->c;
+        need(max_slab_points == 0 || c.n_inst == 1, "chunked runs take one instance (not a batch)");
+        need(max_slab_points == 0 || !(c.flags & NDBX_FORCE_BRUTE), "chunked runs take the cell grid");
+        c.chunk_points = max_slab_points;
+        c.classified = c.labelled = c.meta_valid = false;
+    });
+}
+
+int ndbx_chunk_info(ndbx_ctx *h, int64_t *slabs) {
+    return guarded([&] {
+        need(h != nullptr, "ctx is null");
+        if (slabs) *slabs = h->c.n_slabs;
+    });
 }
 
 int ndbx_dbscan_noalloc(const float *coords, int64_t n, int d, float eps, int min_pts, uint32_t flags,

@@ -1559,6 +1559,10 @@ static void grid_classify_d(Ctx &c) {
             k_dense_tile_fill<<<(unsigned)c.n_dense, 256, 0, s>>>(dv, c.ctile_off.as<int32_t>(), c.ctiles.as<int4>());
     }
     c.nn_s.ensure(n * 4);
+    if (c.no_count) {  // grid and tiles only: counts and core flags are written by the caller
+        c.core_s.ensure(n);
+        return;
+    }
     const bool stats = c.prof.on && c.prof.only.empty() && !cap_mode;  // pair tests when every launch is evented
     if (stats) {
         c.tile_tests.ensure(nct * 8);

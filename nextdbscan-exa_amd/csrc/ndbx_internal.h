@@ -11,6 +11,7 @@
 
 #include "../../include/ndbx.h"
 #include "ndbx_limits.h"
+#include "ndbx_slabs.h"
 
 namespace ndbx {
 
@@ -192,6 +193,15 @@ struct Ctx {
     int64_t audit_band = 0, audit_fp64_flips = 0, audit_filter_flips = 0;
     double audit_max_err = 0.0;
 
+    // chunked runs (ndbx_set_chunking): slabs of at most chunk_points points
+    int64_t chunk_points = 0;
+    int64_t n_slabs = 0;
+    std::vector<Slab> h_slabs;
+    DevBuf ch_perm, ch_grank, ch_parent, ch_gid, ch_coords, ch_ext, ch_inv;
+    struct Ctx *sub = nullptr;      // the per-slab context (grid pipeline buffers for one slab)
+    bool no_count = false;          // grid_classify: grid and tiles only, counts come from outside
+    const int32_t *ext_label = nullptr;  // grid_label: cluster ids of the cores given per point
+
     Profiler prof;
 
     void sync();
@@ -207,6 +217,8 @@ void brute_label(Ctx &c);
 void mfma_classify(Ctx &c);        // d >= 4: partition, bf16 MFMA filter, counts
 void mfma_label(Ctx &c);
 bool mfma_supported(int d);
+void chunked_classify(Ctx &c);     // slab by slab (chunk_points > 0)
+void chunked_label(Ctx &c);
 void cell_table(const uint64_t *keys, int64_t n, int sub_bits, int32_t *flag, int32_t *excl, int32_t *cell_of,
                 int32_t *cell_start, uint64_t *cell_key, int32_t *total, DevBuf &scratch, hipStream_t s,
                 Profiler &prof);

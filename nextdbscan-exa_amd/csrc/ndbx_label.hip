@@ -576,6 +576,20 @@ __global__ __launch_bounds__(256) void k_cell_label(const int32_t *__restrict__ 
     cell_label[c] = rank[root_min[r]] - ig[cell_key[c] >> lin_bits].core_base;
 }
 
+// Cluster id of each core cell from ids given per point (input order) for
+// the cores: the id of the cell's lowest-index core (every core of a cell is
+// in one cluster).  Used in place of the union-find when the clusters were
+// found elsewhere (chunked runs).
+__global__ __launch_bounds__(256) void k_cell_label_ext(const int32_t *__restrict__ cell_has_core,
+                                                        const int32_t *__restrict__ cell_core_min,
+                                                        const int32_t *__restrict__ ncp,
+                                                        const int32_t *__restrict__ ext,
+                                                        int32_t *__restrict__ cell_label) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= *ncp) return;
+    cell_label[c] = cell_has_core[c] ? ext[cell_core_min[c]] : NDBX_NO_CLUSTER;
+}
+
 // Labels per point: cores take their cell's id; a non-core point takes the
 // smallest id among core cells holding a core within eps (its own cell's
 // cores are within eps by construction).  Three launches:
@@ -938,24 +952,32 @@ static void grid_label_d(Ctx &c) {
         k_has_core<<<ncb, 256, 0, s>>>(c.cell_core_min.as<int32_t>(), ncp, c.cell_has_core.as<int32_t>(),
                                        c.core_bits.as<uint64_t>());
     }
+    // clusters given per point by the caller (chunked runs): no union-find,
+    // the border search runs against those ids
+    const int32_t *ext = c.ext_label;
     int32_t *d_total = c.misc.as<int32_t>();
     scan_exclusive_i32(c.label_s.as<int32_t>(), c.rank.as<int32_t>(), n, d_total, c.scratch, s, c.prof);
-    {
+    if (!ext) {
         Launch L(c.prof, s, "core_base");
         k_core_base<<<(unsigned)ceil_div(c.n_inst, 256), 256, 0, s>>>(c.rank.as<int32_t>(), d_total, n, ig, c.n_inst);
     }
     c.parent.ensure(nc * 4);
     c.comp_min.ensure(nc * 4);
     c.cell_label.ensure(nc * 4);
-    {
+    if (!ext) {
         Launch L(c.prof, s, "uf_init");
         k_iota<<<ncb, 256, 0, s>>>(c.parent.as<int32_t>(), nc);
     }
-    fill_i32(c.comp_min.as<int32_t>(), INT_MAX, nc, s, c.prof);
+    if (!ext) fill_i32(c.comp_min.as<int32_t>(), INT_MAX, nc, s, c.prof);
     c.soa_core.ensure((size_t)n * kCoreStride<D> * 4);
     {
         Launch L(c.prof, s, "core_pts");
         k_core_pts<D><<<nb, 256, 0, s>>>(c.soa.as<float>(), c.core_s.as<uint8_t>(), n, npad, c.soa_core.as<float>());
+    }
+    if (ext) {
+        Launch L(c.prof, s, "cell_label");
+        k_cell_label_ext<<<ncb, 256, 0, s>>>(c.cell_has_core.as<int32_t>(), c.cell_core_min.as<int32_t>(), ncp, ext,
+                                             c.cell_label.as<int32_t>());
     }
     // compacted core-cell lists (big: wave per cell; small: lane per cell)
     c.gflag.ensure(nc * 4);
@@ -963,6 +985,7 @@ static void grid_label_d(Ctx &c) {
     int32_t *big = c.gflag.as<int32_t>(), *small = c.ctile_off.as<int32_t>();
     int32_t *counts = c.misc.as<int32_t>() + 8;
     fill_i32(counts, 0, 4, s, c.prof);  // big list, small list, border tiles, border points
+    if (!ext) {
     {
         Launch L(c.prof, s, "cell_lists");
         k_cell_lists<<<(unsigned)ceil_div(nc, 256 * kListPer), 256, 0, s>>>(
@@ -997,6 +1020,7 @@ static void grid_label_d(Ctx &c) {
                                          c.parent.as<int32_t>(), c.comp_min.as<int32_t>(), c.rank.as<int32_t>(),
                                          c.cell_label.as<int32_t>());
     }
+    }  // !ext
     {
         Launch L(c.prof, s, "border");
         const int64_t nct = c.capacity_mode ? c.cap_tiles : c.n_tiles;
@@ -1018,6 +1042,7 @@ static void grid_label_d(Ctx &c) {
             c.soa.as<float>(), c.soa_core.as<float>(), npad, c.core_s.as<uint8_t>(), c.nn_s.as<int32_t>(),
             c.cell_label.as<int32_t>(), c.vals.as<uint32_t>(), c.labels.as<int32_t>());
     }
+    if (ext) return;  // the caller counts clusters
     fill_i32(c.meta.as<int32_t>(), 0, 3 * c.n_inst, s, c.prof);
     {
         Launch L(c.prof, s, "meta");

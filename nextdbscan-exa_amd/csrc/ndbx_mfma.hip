@@ -749,15 +749,22 @@ constexpr int kBandQ = 64;       // guard-band pairs queued per wave in LDS
 // with 0.40 TB; 12 (3 per SIMD) 234 ms with 0.30 TB: the pass is bound by
 // issue and latency, not by its candidate-block traffic.
 constexpr int kCountWaves = 4;
+// Waves per SIMD the count pass is built for (d <= 64).
+#ifndef NDBX_MF_COUNT_OCC
+#define NDBX_MF_COUNT_OCC 4
+#endif
+constexpr int kCountOcc = NDBX_MF_COUNT_OCC;
 
 template <int KP, int WAVES>
-__global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? 3 : 2) void k_mf_count(MfmaArgs A, int chunk) {
+__global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 2) void k_mf_count(MfmaArgs A, int chunk) {
     constexpr int KS = KP / 16;
     constexpr int UNITS = 32 * (KP / 8);  // bf16x8 units of a block, per hi / lo
     constexpr int ROWU = KP / 8 + 1;
     constexpr int NT = 64 * WAVES;
-    constexpr int UPT = (2 * UNITS + NT - 1) / NT;  // hi and lo units per thread
-    __shared__ bf16x8_t sb_hi[2][32 * ROWU], sb_lo[2][32 * ROWU];
+    constexpr int UPT = (UNITS + NT - 1) / NT;  // hi units per thread
+    // only the hi halves are staged: the lo halves are read from global
+    // memory by the few blocks that need the full products
+    __shared__ bf16x8_t sb_hi[2][32 * ROWU];
     __shared__ float sb_na[2][32];
     __shared__ int32_t sb_cacc[kColWin];  // column counts of the current window
     __shared__ int32_t sb_edge;
@@ -774,15 +781,13 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? 3 : 2) void 
     const bf16x8_t *hi8 = reinterpret_cast<const bf16x8_t *>(A.hi);
     const bf16x8_t *lo8 = reinterpret_cast<const bf16x8_t *>(A.lo);
     const float INF = __builtin_inff();
-    // A operand: row `col` of the tile, k elements [16 ks + 8 half, +8)
+    // A operand: row `col` of the tile, k elements [16 ks + 8 half, +8);
+    // the lo halves are read again by the blocks that need them
     const bool arow_ok = col < nrows;
     const int64_t arow = row0 + (arow_ok ? col : 0);
-    bf16x8_t ahi[KS], alo[KS];
+    bf16x8_t ahi[KS];
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-        ahi[ks] = arow_ok ? hi8[arow * (KP / 8) + ks * 2 + half] : bf16x8_t{};
-        alo[ks] = arow_ok ? lo8[arow * (KP / 8) + ks * 2 + half] : bf16x8_t{};
-    }
+    for (int ks = 0; ks < KS; ++ks) ahi[ks] = arow_ok ? hi8[arow * (KP / 8) + ks * 2 + half] : bf16x8_t{};
     // accumulator rows: r -> tile row 8 (r/4) + 4 half + (r%4); rows past the
     // tile have mid = +inf, so dv = -inf: never counted, never in the band
     float mid_r[16];
@@ -849,11 +854,9 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? 3 : 2) void 
 #pragma unroll
         for (int q = 0; q < UPT; ++q) {
             const int u = threadIdx.x + q * NT;
-            const int uu = u < UNITS ? u : u - UNITS;  // units [0, UNITS): hi, then lo
-            const int bc = uu / (KP / 8), kk = uu - bc * (KP / 8);
+            const int bc = u / (KP / 8), kk = u - bc * (KP / 8);
             const int64_t pt = cb + bc < n_last ? cb + bc : n_last;
-            const int64_t src = pt * (KP / 8) + kk;
-            if (u < 2 * UNITS) pf[q] = u < UNITS ? hi8[src] : lo8[src];
+            if (u < UNITS) pf[q] = hi8[pt * (KP / 8) + kk];
         }
         const int jn = cb + (int)(threadIdx.x & 31);
         pf_na = A.na[jn < n_last ? jn : n_last];
@@ -863,10 +866,8 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? 3 : 2) void 
 #pragma unroll
         for (int q = 0; q < UPT; ++q) {
             const int u = threadIdx.x + q * NT;
-            const int uu = u < UNITS ? u : u - UNITS;
-            const int bc = uu / (KP / 8), kk = uu - bc * (KP / 8);
+            const int bc = u / (KP / 8), kk = u - bc * (KP / 8);
             if (u < UNITS) sb_hi[buf][bc * ROWU + kk] = pf[q];
-            else if (u < 2 * UNITS) sb_lo[buf][bc * ROWU + kk] = pf[q];
         }
         if (threadIdx.x < 32) sb_na[buf][threadIdx.x] = pf_ok ? pf_na : INF;
     };
@@ -878,7 +879,7 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? 3 : 2) void 
     publish(0);
     __syncthreads();
     int buf = 0, cur_o = -1;
-    float hb = 0.0f;
+    float hb = 0.0f, hskip = 0.0f;
     for (;;) {
         // this block's place in the sequence (wave-uniform, in scalar
         // registers), then the next block's loads, which overlap this
@@ -894,6 +895,13 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? 3 : 2) void 
                 cur_o = o;
                 const double R = fmax((double)rmax_row, (double)ld_const(A.cell_rmax + o));
                 hb = (float)(A.b1 * R * R + A.b2 * R + A.b3);  // half the band
+                // skip threshold of the hi*hi partial sum: the two products
+                // it leaves out are each at most 2^-8 (1 + 2^-8)^2 |a| |b|
+                // (|lo| <= 2^-8 |x| componentwise), 0.008 R^2 together; its
+                // own float32 accumulation error and the full sum's are each
+                // within hb, so a partial below -(3 hb + 0.008 R^2) leaves the
+                // full dv below -hb
+                hskip = -(float)(3.0 * (double)hb + 0.008 * R * R);
             }
             // the accumulator starts at -|b|^2/2 - (|a|^2 - e2)/2 and ends at
             // dv = (e2 - est)/2 (-inf for rows and columns out of range)
@@ -901,13 +909,27 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? 3 : 2) void 
             f32x16_t acc;
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[r] = nbh - mid_r[r];
+            // hi*hi first: a block whose partial largest dv is below hskip
+            // holds no neighbour and no band element under the full
+            // products, and costs 2 MFMAs per 16-wide k step less
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
                 const bf16x8_t bhi = sb_hi[buf][col * ROWU + ks * 2 + half];
-                const bf16x8_t blo = sb_lo[buf][col * ROWU + ks * 2 + half];
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[ks], bhi, acc, 0, 0, 0);
+            }
+            float hmax = acc[0];
+#pragma unroll
+            for (int r = 1; r < 16; ++r) hmax = fmaxf(hmax, acc[r]);
+            int32_t ccnt = 0;
+            if (__ballot(hmax >= hskip) != 0) {
+            const int64_t bpt = bcb + col < n_last ? bcb + col : n_last;  // clamped as in fetch
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const bf16x8_t blo = lo8[bpt * (KP / 8) + ks * 2 + half];
+                const bf16x8_t alo = arow_ok ? lo8[arow * (KP / 8) + ks * 2 + half] : bf16x8_t{};
+                const bf16x8_t bhi = sb_hi[buf][col * ROWU + ks * 2 + half];
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[ks], blo, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[ks], bhi, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi, acc, 0, 0, 0);
             }
             // dv >= hb is a neighbour, dv < -hb is not, the rest (the guard
             // band) the exact predicate decides.  A block whose largest dv is
@@ -915,7 +937,6 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? 3 : 2) void 
             float amax = acc[0];
 #pragma unroll
             for (int r = 1; r < 16; ++r) amax = fmaxf(amax, acc[r]);
-            int32_t ccnt = 0;
             if (__ballot(amax >= -hb) != 0) {
             float amin = INF;
 #pragma unroll
@@ -973,6 +994,7 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? 3 : 2) void 
             if (!diag) {
                 ccnt += __shfl_xor(ccnt, 32, 64);
                 if (half == 0 && ccnt != 0) atomicAdd(&sb_cacc[(bcb - bcs) % kColWin + col], ccnt);
+            }
             }
             }
         }

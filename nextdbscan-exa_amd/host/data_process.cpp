@@ -24,6 +24,9 @@ data_process::data_process(h_vec<float> &v_coord, int const m, float const e, in
     uint32_t flags = is_approximate ? NDBX_APPROX : 0u;
     if (std::getenv("NDBX_FORCE_BRUTE")) flags |= NDBX_FORCE_BRUTE;
     check(ndbx_create(&ctx, v_coord.data(), (int64_t)n_coord, n_dim, e, m, flags), "ndbx_create");
+    // slab-by-slab runs with at most this many points per slab
+    if (const char *cp = std::getenv("NDBX_CHUNK_POINTS"))
+        check(ndbx_set_chunking(ctx, std::atoll(cp)), "ndbx_set_chunking");
     this->v_coord = std::move(v_coord);
     allocated_bytes = (unsigned long long)n_coord * (unsigned long long)n_dim * sizeof(float);
 }

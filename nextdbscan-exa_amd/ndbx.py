@@ -62,6 +62,8 @@ SIGNATURES = {
     "ndbx_batch_meta": ([_P, _P], ctypes.c_int),
     "ndbx_batch_set_coords": ([_P, _P], ctypes.c_int),
     "ndbx_batch_info": ([_P, _IP, _I64P, _I64P], ctypes.c_int),
+    "ndbx_set_chunking": ([_P, _I64], ctypes.c_int),
+    "ndbx_chunk_info": ([_P, _I64P], ctypes.c_int),
     "ndbx_dbscan_noalloc": ([_P, _I64, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_uint32, _P, _P, _P,
                              _P], ctypes.c_int),
 }
@@ -160,6 +162,16 @@ class Context:
 
     def sync(self):
         _check(lib().ndbx_sync(self._h))
+
+    def set_chunking(self, max_slab_points):
+        """Slab-by-slab runs with at most max_slab_points points per slab (0: resident)."""
+        _check(lib().ndbx_set_chunking(self._h, int(max_slab_points)))
+        return self
+
+    def slabs(self):
+        k = ctypes.c_int64()
+        _check(lib().ndbx_chunk_info(self._h, ctypes.byref(k)))
+        return k.value
 
     def stats(self):
         a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()

@@ -150,3 +150,20 @@ def test_host_data_process_members(gpu_lib, tmp_path, name):
     assert n_lab == core.sum() == cores
     np.testing.assert_array_equal(lab, g["cci"][core])
     assert roots == clusters == g["meta"][2] and (cores, noise) == g["meta"][:2]
+
+
+@pytest.mark.gpu
+def test_cli_chunked_run(gpu_lib, tmp_path):
+    """NDBX_CHUNK_POINTS runs the CLI slab by slab: the same printed triple
+    and per-point labels as the reference."""
+    g = load_golden("skew2d_20k")
+    p, out = str(tmp_path / "in.bin"), str(tmp_path / "labels.txt")
+    synth.write_bin(p, g["x"])
+    r = subprocess.run([CLI, "-m", str(g["m"]), "-e", g["eps_text"], "-o", out, p], capture_output=True, text=True,
+                       timeout=120, env=dict(os.environ, NDBX_CHUNK_POINTS="6000"))
+    assert r.returncode == 0, r.stderr
+    cl = int(re.search(r"Estimated Clusters: (-?\d+)", r.stdout).group(1))
+    co = int(re.search(r"Core Points: (-?\d+)", r.stdout).group(1))
+    no = int(re.search(r"Noise Points: (-?\d+)", r.stdout).group(1))
+    assert [cl, co, no] == TRIPLES["skew2d_20k"]
+    np.testing.assert_array_equal(np.loadtxt(out, dtype=np.int64).reshape(-1), g["cci"])

@@ -23,3 +23,8 @@ def build_and_run(tmp_path, name):
 def test_mfma_pair_limit(tmp_path):
     out = build_and_run(tmp_path, "limits_check")
     assert "FAIL" not in out and out.count("ok ") >= 8
+
+
+def test_slab_plan(tmp_path):
+    out = build_and_run(tmp_path, "slabs_check")
+    assert "FAIL" not in out and "ok 300 random plans" in out
