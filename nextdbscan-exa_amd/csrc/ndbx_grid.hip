@@ -1004,26 +1004,17 @@ __device__ __forceinline__ int32_t count_run_sym(const float (&qx)[D], const flo
 // on each lane and loop over their queries: the candidate's count stays in
 // its lane and each query's count is a popcount of the compare mask.
 template <int D>
-// 8 waves per SIMD (SGPRs capped, the rest spilled to VGPR lanes): 7 waves
-// left the tiles' load chains exposed (10M 3-D 12.9 -> 12.0 ms, batch 23.9 ->
-// 21.0 ms).
-__global__ __launch_bounds__(256, 8) void k_count_sym(const int4 *__restrict__ ctiles, const int32_t *__restrict__ ntp,
-                                                   const int32_t *__restrict__ cell_start,
-                                                   const int32_t *__restrict__ cell_of,
-                                                   const uint64_t *__restrict__ cell_key,
-                                                   const InstGrid *__restrict__ ig, int lin_bits,
-                                                   const int2 *__restrict__ ranges, int rows_per_cell,
-                                                   const int32_t *__restrict__ cell_c, const int32_t *__restrict__ ncp,
-                                                   const float *__restrict__ soa, int64_t npad,
-                                                   int32_t *__restrict__ nn_s, int64_t *__restrict__ tests,
-                                                   const int32_t *__restrict__ part_fine,
-                                                   const int32_t *__restrict__ big_id) {
-    __shared__ float4 qsh[4][16];  // per wave: the queries of a small tile
-    const int64_t nc = *ncp;
-    const int64_t n_tiles = *ntp;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
-    if (t >= n_tiles) return;
+__device__ __forceinline__ void count_tile_sym(int64_t t, const int4 *__restrict__ ctiles,
+                                               const int32_t *__restrict__ cell_start,
+                                               const int32_t *__restrict__ cell_of,
+                                               const uint64_t *__restrict__ cell_key,
+                                               const InstGrid *__restrict__ ig, int lin_bits,
+                                               const int2 *__restrict__ ranges, int rows_per_cell,
+                                               const int32_t *__restrict__ cell_c, int64_t nc,
+                                               const float *__restrict__ soa, int64_t npad,
+                                               int32_t *__restrict__ nn_s, int64_t *__restrict__ tests,
+                                               const int32_t *__restrict__ part_fine,
+                                               const int32_t *__restrict__ big_id, float4 *__restrict__ qsh) {
     const int lane = threadIdx.x & 63;
     const int4 tile = ctiles[t];
     const int cf = tile.x, cl = tile.y;
@@ -1184,7 +1175,7 @@ __global__ __launch_bounds__(256, 8) void k_count_sym(const int4 *__restrict__ c
         const int total = __builtin_amdgcn_readlane(incl, 63);
         // the queries (coordinates, end of the query's own cell) for broadcast
         // reads: one 16-byte LDS read per query and round
-        if (lane < 16) qsh[wave][lane] = make_float4(qx[0], D >= 2 ? qx[D >= 2 ? 1 : 0] : 0.0f,
+        if (lane < 16) qsh[lane] = make_float4(qx[0], D >= 2 ? qx[D >= 2 ? 1 : 0] : 0.0f,
                                                      D >= 3 ? qx[D >= 3 ? 2 : 0] : 0.0f, __int_as_float(my_end));
         __builtin_amdgcn_wave_barrier();
         int32_t qcnt[16];
@@ -1223,7 +1214,7 @@ __global__ __launch_bounds__(256, 8) void k_count_sym(const int4 *__restrict__ c
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
                 if (i < tile.w) {
-                    const float4 qv = qsh[wave][i];
+                    const float4 qv = qsh[i];
                     const float qq[3] = {qv.x, qv.y, qv.z};
                     float l2 = 0.0f;
 #pragma unroll
@@ -1245,6 +1236,202 @@ __global__ __launch_bounds__(256, 8) void k_count_sym(const int4 *__restrict__ c
     }
     if (valid) atomicAdd(nn_s + q, cnt + (my_end - my_beg));
     if (tests && lane == 0) tests[t] = issued;
+}
+
+// Four tiles of at most 16 queries in one wave: each 16-lane group takes a
+// tile.  The lanes of a group compute its windows (the 13 stencil rows from
+// the centre row on in 3-D, and the tile's own cells after its first), then
+// take 16 candidates at a time and test them against the group's queries
+// in turn; each lane keeps per-query partial counts, summed over the group
+// at the end.  On sparse grids (a few points per tile) the time of a tile
+// is its chain of dependent loads (tile, points, instance, windows, cell
+// ranges, candidates); four tiles per wave run four such chains at once.
+template <int D>
+__device__ __forceinline__ void count_quad_sym(int64_t t0, int64_t n_tiles, const int4 *__restrict__ ctiles,
+                                               const int32_t *__restrict__ cell_start,
+                                               const int32_t *__restrict__ cell_of,
+                                               const uint64_t *__restrict__ cell_key,
+                                               const InstGrid *__restrict__ ig, int lin_bits,
+                                               const int2 *__restrict__ ranges, int rows_per_cell,
+                                               const int32_t *__restrict__ cell_c, int64_t nc,
+                                               const float *__restrict__ soa, int64_t npad,
+                                               int32_t *__restrict__ nn_s, int64_t *__restrict__ tests,
+                                               float4 *__restrict__ qsh) {
+    constexpr int R = kStencilRows<D>, C0 = R / 2, OWN = R - C0;  // lanes [0, OWN): rows C0.., lane OWN: own cells
+    const int lane = threadIdx.x & 63, g = lane >> 4, gl = lane & 15;
+    const int64_t t = t0 + g;
+    const bool tv = t < n_tiles;
+    const int4 tile = tv ? ctiles[t] : make_int4(0, 0, 0, 0);
+    const int cf = tile.x, cl = tile.y, wq = tile.w;
+    const bool valid = gl < wq;
+    const int q = tile.z + (valid ? gl : 0);
+    float qx[D], lo[D], hi[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        qx[k] = soa[k * npad + q];
+        float a = valid ? qx[k] : __builtin_inff(), b = valid ? qx[k] : -__builtin_inff();
+#pragma unroll
+        for (int o = 8; o >= 1; o >>= 1) {
+            a = fminf(a, __shfl_xor(a, o, 16));
+            b = fmaxf(b, __shfl_xor(b, o, 16));
+        }
+        lo[k] = a;
+        hi[k] = b;
+    }
+    const int my_cell = cell_of[q];
+    const int my_beg = cell_start[my_cell], my_end = cell_start[my_cell + 1];
+    const InstGrid &gg = ig[cell_key[cf] >> lin_bits];
+    const float e2 = gg.e2;
+    int2 w = make_int2(0, 0);
+    if (wq > 0 && gl < OWN) {
+        w = tile_window<D>(C0 + gl, rows_per_cell, cf, cl, lo, hi, gg, D >= 2 ? cell_c[nc + cf] : 0,
+                           D >= 3 ? cell_c[2 * nc + cf] : 0, ranges, cell_c, nc);
+        if (gl == 0 && w.x < cl + 1) w.x = cl + 1;  // centre row: the cells after the tile's last
+        if (w.x >= w.y) w = make_int2(0, 0);
+    }
+    int ps = w.x < w.y ? cell_start[w.x] : 0, pe = w.x < w.y ? cell_start[w.y] : 0;
+    if (gl == OWN && wq > 0) {  // the tile's own cells after its first
+        ps = cell_start[cf + 1];
+        pe = cl > cf ? cell_start[cl + 1] : ps;
+    }
+    const int len = pe - ps;
+    int incl = len;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+        const int t2 = __shfl_up(incl, o, 16);
+        if (gl >= o) incl += t2;
+    }
+    const int pre = incl - len;
+    const int total = __shfl(incl, 15, 16);
+    int tmax = total;
+    for (int o = 32; o >= 16; o >>= 1) tmax = max(tmax, __shfl_xor(tmax, o, 64));
+    tmax = __builtin_amdgcn_readfirstlane(tmax);
+    if (valid) qsh[lane] = make_float4(qx[0], D >= 2 ? qx[D >= 2 ? 1 : 0] : 0.0f, D >= 3 ? qx[D >= 3 ? 2 : 0] : 0.0f,
+                                       __int_as_float(my_end));
+    __builtin_amdgcn_wave_barrier();
+    // candidate gc of the group's flattened windows: its point index (the
+    // last lane whose window starts at or before gc holds it)
+    auto cand_of = [&](int cb0) {
+        const int gc = cb0 + gl < total ? cb0 + gl : (total > 0 ? total - 1 : 0);
+        int lo_w = 0, hi_w = 15;
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+            const int mid = (lo_w + hi_w + 1) >> 1;
+            if (__shfl(pre, mid, 16) <= gc) lo_w = mid;
+            else hi_w = mid - 1;
+        }
+        return __shfl(ps, lo_w, 16) + gc - __shfl(pre, lo_w, 16);
+    };
+    int32_t qp[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) qp[i] = 0;
+    int jn = cand_of(0);
+    float cn[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) cn[k] = soa[k * npad + jn];
+    for (int cb = 0; cb < tmax; cb += 16) {
+        const bool okc = cb + gl < total;
+        const int j = jn;
+        float c[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) c[k] = cn[k];
+        if (cb + 16 < tmax) {  // the next round's candidates load during this round's tests
+            jn = cand_of(cb + 16);
+#pragma unroll
+            for (int k = 0; k < D; ++k) cn[k] = soa[k * npad + jn];
+        }
+        int32_t cc = 0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            if (i < wq) {
+                const float4 qv = qsh[g * 16 + i];
+                const float qq[3] = {qv.x, qv.y, qv.z};
+                float l2 = 0.0f;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const float df = qq[k] - c[k];
+                    l2 = __fadd_rn(l2, __fmul_rn(df, df));
+                }
+                const bool ok = okc & (l2 <= e2) & (j >= __float_as_int(qv.w));
+                cc = add_if(cc, ok);
+                qp[i] = add_if(qp[i], ok);
+            }
+        }
+        if (okc && cc != 0) atomicAdd(nn_s + j, cc);
+    }
+    // each query's count: its partials summed over the group's lanes
+    int32_t cnt = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        int32_t v = qp[i];
+#pragma unroll
+        for (int o = 8; o >= 1; o >>= 1) v += __shfl_xor(v, o, 16);
+        if (gl == i) cnt = v;
+    }
+    if (valid) atomicAdd(nn_s + q, cnt + (my_end - my_beg));
+    if (tests && tv && gl == 0) tests[t] = (int64_t)total * wq;
+}
+
+// Whether the four tiles [t0, t0 + 4) all hold at most 16 queries (tiles
+// past the end count as such): those run side by side in k_count_quad.
+__device__ __forceinline__ bool quad_small(const int4 *__restrict__ ctiles, int64_t t0, int64_t n_tiles) {
+    const int lane = threadIdx.x & 63;
+    const int w4 = lane < 4 && t0 + lane < n_tiles ? ctiles[t0 + lane].w : 0;
+    return __ballot(w4 > 16) == 0;
+}
+
+// One wave per count tile, over the higher-index candidates only; with
+// `quads`, tiles of quads of small tiles are left to k_count_quad.  Tiles of more than 16
+// queries walk the windows wave-uniformly (scalar candidate loads, one query
+// per lane; cells of 64+ points by the parts in reach of a tile box at most
+// kPartBoxCells wide).  Smaller tiles put one candidate on each lane and loop
+// over their queries: the candidate's count stays in its lane and each
+// query's count is a popcount of the compare mask.
+template <int D>
+// 8 waves per SIMD (SGPRs capped, the rest spilled to VGPR lanes): 7 waves
+// left the tiles' load chains exposed (10M 3-D 12.9 -> 12.0 ms, batch 23.9 ->
+// 21.0 ms).
+__global__ __launch_bounds__(256, 8) void k_count_sym(const int4 *__restrict__ ctiles, const int32_t *__restrict__ ntp,
+                                                   const int32_t *__restrict__ cell_start,
+                                                   const int32_t *__restrict__ cell_of,
+                                                   const uint64_t *__restrict__ cell_key,
+                                                   const InstGrid *__restrict__ ig, int lin_bits,
+                                                   const int2 *__restrict__ ranges, int rows_per_cell,
+                                                   const int32_t *__restrict__ cell_c, const int32_t *__restrict__ ncp,
+                                                   const float *__restrict__ soa, int64_t npad,
+                                                   int32_t *__restrict__ nn_s, int64_t *__restrict__ tests,
+                                                   const int32_t *__restrict__ part_fine,
+                                                   const int32_t *__restrict__ big_id, int quads) {
+    __shared__ float4 qsh[4][16];  // per wave: the queries of a small tile
+    const int64_t nc = *ncp;
+    const int64_t n_tiles = *ntp;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t t = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+    if (t >= n_tiles || (quads && quad_small(ctiles, t & ~(int64_t)3, n_tiles))) return;
+    count_tile_sym<D>(t, ctiles, cell_start, cell_of, cell_key, ig, lin_bits, ranges, rows_per_cell, cell_c, nc, soa,
+                      npad, nn_s, tests, part_fine, big_id, qsh[wave]);
+}
+
+// One wave per four consecutive count tiles that all hold at most 16
+// queries (count_quad_sym); other quads are k_count_sym's.
+template <int D>
+__global__ __launch_bounds__(256, 8) void k_count_quad(const int4 *__restrict__ ctiles, const int32_t *__restrict__ ntp,
+                                                    const int32_t *__restrict__ cell_start,
+                                                    const int32_t *__restrict__ cell_of,
+                                                    const uint64_t *__restrict__ cell_key,
+                                                    const InstGrid *__restrict__ ig, int lin_bits,
+                                                    const int2 *__restrict__ ranges, int rows_per_cell,
+                                                    const int32_t *__restrict__ cell_c, const int32_t *__restrict__ ncp,
+                                                    const float *__restrict__ soa, int64_t npad,
+                                                    int32_t *__restrict__ nn_s, int64_t *__restrict__ tests) {
+    __shared__ float4 qsh[4][64];  // per wave: the queries of its four tiles
+    const int64_t nc = *ncp;
+    const int64_t n_tiles = *ntp;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t t0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + wave) * 4;
+    if (t0 >= n_tiles || !quad_small(ctiles, t0, n_tiles)) return;
+    count_quad_sym<D>(t0, n_tiles, ctiles, cell_start, cell_of, cell_key, ig, lin_bits, ranges, rows_per_cell, cell_c,
+                      nc, soa, npad, nn_s, tests, qsh[wave]);
 }
 
 // Sum of int64 values (pair-test statistics): one atomic per block.
@@ -1581,10 +1768,19 @@ static void grid_classify_d(Ctx &c) {
         // each pair of distinct cells tested once, counts added to both points
         fill_i32(c.nn_s.as<int32_t>(), 0, n, s, c.prof);
         Launch L(c.prof, s, "count");
+        // sparse grids (fewer than 8 points per tile on average): quads of
+        // small tiles share a wave (4-instance batch of 50M 3-D points: count
+        // 21.2 -> 18.4 ms); on denser grids the second launch only costs
+        // (10M 3-D: 11.4 -> 11.9 ms), so it is left out
+        const int quads = c.n_tiles > 0 && c.n < 8 * c.n_tiles;
         k_count_sym<D><<<(unsigned)ceil_div(nct, 4), 256, 0, s>>>(
             c.ctiles.as<int4>(), ntp, c.cell_start.as<int32_t>(), c.cell_of.as<int32_t>(), c.cell_key.as<uint64_t>(),
             ig, c.lin_bits, c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), ncp, c.soa.as<float>(), npad,
-            c.nn_s.as<int32_t>(), stats ? c.tile_tests.as<int64_t>() : nullptr, part_fine, big_id);
+            c.nn_s.as<int32_t>(), stats ? c.tile_tests.as<int64_t>() : nullptr, part_fine, big_id, quads);
+        if (quads) k_count_quad<D><<<(unsigned)ceil_div(nct, 16), 256, 0, s>>>(
+            c.ctiles.as<int4>(), ntp, c.cell_start.as<int32_t>(), c.cell_of.as<int32_t>(), c.cell_key.as<uint64_t>(),
+            ig, c.lin_bits, c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), ncp, c.soa.as<float>(), npad,
+            c.nn_s.as<int32_t>(), stats ? c.tile_tests.as<int64_t>() : nullptr);
     }
     if (stats) {
         Launch L(c.prof, s, "stats");
