@@ -101,6 +101,7 @@ __global__ __launch_bounds__(64) void k_plan(const BoundsPart *__restrict__ part
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_inst) return;
     InstGrid g;
+    g.row_base = 0;  // captured runs use the row hash
     const float e = eps[i];
     const float e2 = __fmul_rn(e, e);  // data_process.h:73
     g.e2 = e2;
@@ -247,6 +248,7 @@ void grid_plan(Ctx &c) {
         g.point_begin = c.h_offsets[i];
         g.point_end = c.h_offsets[i + 1];
         g.core_base = 0;
+        g.row_base = 0;
         for (int k = 0; k < 3; ++k) {
             g.origin[k] = 0.0;
             g.G[k] = 1;
@@ -291,6 +293,15 @@ void grid_plan(Ctx &c) {
         c.path = ok ? PATH_MFMA : PATH_BRUTE;
     }
     if (c.flags & NDBX_FORCE_BRUTE) c.path = PATH_BRUTE;
+    // rows (c2, c1) of every instance's grid, one after another: a direct
+    // row table is indexed by row_base + c2 G1 + c1
+    c.n_rows_direct = 0;
+    for (int i = 0; i < c.n_inst; ++i) {
+        InstGrid &g = c.h_inst[i];
+        g.row_base = c.n_rows_direct;
+        const int64_t rows = g.G[1] * g.G[2];
+        c.n_rows_direct = rows > ((int64_t)1 << 40) - c.n_rows_direct ? ((int64_t)1 << 40) : c.n_rows_direct + rows;
+    }
     const int inst_bits = bits_for((uint64_t)c.n_inst);
     c.lin_bits = lin_bits;
     c.cell_bits = lin_bits + inst_bits;
@@ -482,11 +493,34 @@ __device__ __forceinline__ bool row_lookup(const RowSlot *__restrict__ table, ui
 // sorted order, and so are their points, [cell_start[a], cell_start[e]).  An
 // empty window keeps its insertion position (a == e), so the window of a run
 // of cells [cf, cl] of one row is [slot(cf).a, slot(cl).e).
-template <int D>
+// Direct row table (eager runs whose grids have few enough rows): entry
+// row_base + c2 G1 + c1 of the cell's instance holds its row's [first, end)
+// cells, (0, 0) when the row is empty.  A cell's 5 rows of one c2 are
+// adjacent entries, and cells of one row read the same ones.
+__global__ __launch_bounds__(256) void k_row_insert_direct(const uint64_t *__restrict__ cell_key,
+                                                           const int32_t *__restrict__ ncp,
+                                                           const InstGrid *__restrict__ ig, int lin_bits,
+                                                           const int32_t *__restrict__ cell_c,
+                                                           int2 *__restrict__ table) {
+    const int64_t nc = *ncp;
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const uint64_t rk = cell_key[c] - (uint64_t)cell_c[c];
+    const bool first = c == 0 || cell_key[c - 1] - (uint64_t)cell_c[c - 1] != rk;
+    const bool last = c == nc - 1 || cell_key[c + 1] - (uint64_t)cell_c[c + 1] != rk;
+    if (!first && !last) return;
+    const InstGrid &g = ig[cell_key[c] >> lin_bits];
+    const int64_t row = g.row_base + (int64_t)cell_c[2 * nc + c] * g.G[1] + cell_c[nc + c];
+    if (first) table[row].x = (int32_t)c;
+    if (last) table[row].y = (int32_t)(c + 1);
+}
+
+template <int D, bool DIRECT>
 __global__ __launch_bounds__(256) void k_stencil(const uint64_t *__restrict__ cell_key,
                                                  const int32_t *__restrict__ cell_start, const int32_t *__restrict__ ncp,
                                                  const InstGrid *__restrict__ ig, int lin_bits,
                                                  const RowSlot *__restrict__ table, uint64_t mask,
+                                                 const int2 *__restrict__ dtable,
                                                  int2 *__restrict__ ranges, const int32_t *__restrict__ cell_c) {
     constexpr int R = kStencilRows<D>, k = 2;
     const int64_t nc = *ncp;
@@ -509,7 +543,14 @@ __global__ __launch_bounds__(256) void k_stencil(const uint64_t *__restrict__ ce
     int2 rv;
     if (z >= 0 && z < G2 && y >= 0 && y < G1) {
         const uint64_t rk = (inst << lin_bits) | (uint64_t)((z * G1 + y) * G0);
-        if (row_lookup(table, mask, rk, &rv)) {
+        bool found;
+        if (DIRECT) {
+            rv = dtable[g.row_base + z * G1 + y];
+            found = rv.x < rv.y;
+        } else {
+            found = row_lookup(table, mask, rk, &rv);
+        }
+        if (found) {
             int a = rv.x, b = rv.y;  // first cell with key >= rk+lo0
             const uint64_t klo = rk + (uint64_t)lo0, khi = rk + (uint64_t)hi0;
             while (a < b) {
@@ -1700,9 +1741,18 @@ static void grid_classify_d(Ctx &c) {
         Launch L(c.prof, s, "cell_coords");
         k_cell_coords<<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), ncp, ig, c.lin_bits, c.cell_c.as<int32_t>());
     }
-    c.row_keys.ensure(cap * sizeof(RowSlot));
-    fill_u64(c.row_keys.as<uint64_t>(), kEmpty, (int64_t)cap * 2, s, c.prof);  // keys (and ranges) all ones
-    {
+    // eager runs take a direct row table when it is no larger than the hash
+    // (8-byte entries, at most 2 per hash slot); captured runs the hash
+    const bool direct = !cap_mode && c.n_rows_direct <= 2 * (int64_t)cap;
+    if (direct) {
+        c.row_keys.ensure((size_t)c.n_rows_direct * sizeof(int2));
+        fill_u64(c.row_keys.as<uint64_t>(), 0, c.n_rows_direct, s, c.prof);  // (0, 0): empty row
+        Launch L(c.prof, s, "row_insert");
+        k_row_insert_direct<<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), ncp, ig, c.lin_bits,
+                                                c.cell_c.as<int32_t>(), c.row_keys.as<int2>());
+    } else {
+        c.row_keys.ensure(cap * sizeof(RowSlot));
+        fill_u64(c.row_keys.as<uint64_t>(), kEmpty, (int64_t)cap * 2, s, c.prof);  // keys (and ranges) all ones
         Launch L(c.prof, s, "row_insert");
         k_row_insert<<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), ncp, c.cell_c.as<int32_t>(),
                                          c.row_keys.as<RowSlot>(), cap - 1);
@@ -1711,9 +1761,14 @@ static void grid_classify_d(Ctx &c) {
     c.ranges.ensure((size_t)nc * rows * 8);
     {
         Launch L(c.prof, s, "stencil");
-        k_stencil<D><<<(unsigned)ceil_div(nc * rows, 256), 256, 0, s>>>(
-            c.cell_key.as<uint64_t>(), c.cell_start.as<int32_t>(), ncp, ig, c.lin_bits, c.row_keys.as<RowSlot>(),
-            cap - 1, c.ranges.as<int2>(), c.cell_c.as<int32_t>());
+        if (direct)
+            k_stencil<D, true><<<(unsigned)ceil_div(nc * rows, 256), 256, 0, s>>>(
+                c.cell_key.as<uint64_t>(), c.cell_start.as<int32_t>(), ncp, ig, c.lin_bits, nullptr, 0,
+                c.row_keys.as<int2>(), c.ranges.as<int2>(), c.cell_c.as<int32_t>());
+        else
+            k_stencil<D, false><<<(unsigned)ceil_div(nc * rows, 256), 256, 0, s>>>(
+                c.cell_key.as<uint64_t>(), c.cell_start.as<int32_t>(), ncp, ig, c.lin_bits, c.row_keys.as<RowSlot>(),
+                cap - 1, nullptr, c.ranges.as<int2>(), c.cell_c.as<int32_t>());
     }
     // count tiles over groups of cells of one row
     c.gflag.ensure(nc * 4);

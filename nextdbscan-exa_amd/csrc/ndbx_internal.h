@@ -109,6 +109,7 @@ struct InstGrid {
     int32_t m;
     int32_t reach;  // stencil half-width in cells
     int32_t core_base;  // cores before this instance (input order), set on device
+    int64_t row_base;   // first entry of the instance's rows in a direct row table (eager runs)
 };
 
 // Per-chunk bounds partials: min[3], max[3], special count.
@@ -167,6 +168,7 @@ struct Ctx {
         core_s, cell_core_min, cell_has_core, core_bits, border_tiles, parent, comp_min, cell_label, label_s, rank,
         misc, scratch, scratch2, soa_core, cell_c, gflag, ctile_off, ctiles, tile_tests, dense_id, dense_info, dense_tmp, fine_off, part_off, part_fine, big_id;
     int64_t n_cells = 0, n_tiles = 0, n_label_tiles = 0, pair_tests = 0, n_dense = 0;
+    int64_t n_rows_direct = 0;  // rows of all instances' (c2, c1) grids (eager runs)
     int lin_bits = 0, cell_bits = 0, sub_bits = 0, key_bits = 0, reach_max = 2, rows_per_cell = 1;
     int group_cells = 8;  // cells of one row per counting group
     bool tuned = false;   // group_cells chosen by the autotuner
