@@ -376,9 +376,11 @@ def bench_batch(n_inst, steps, warmup, cpu=True, threads=16):
         torch.cuda.synchronize()
         g_ms = (time.perf_counter() - t0) * 1e3 / steps
         gc, replays, reruns = gb.info()
+        tw = gb.tile_shape()[1]
         same = bool(np.array_equal(gmeta, bmeta) and np.array_equal(gb.nn(), nn))
     out["graph"] = {"ms_per_step": g_ms, "value": n / (g_ms * 1e-3), "replays": replays, "reruns": reruns,
-                    "group_cells": gc, "same_result": same, "form": "graph replay, autotuned tile shape"}
+                    "group_cells": gc, "tiles_per_wave": tw, "same_result": same,
+                    "form": "graph replay, autotuned tile shape"}
     assert same, "graph replay differs from the eager batched run"
     # plain form: one context per instance, device-resident points
     x_dev = torch.from_numpy(x).to("cuda")

@@ -186,6 +186,13 @@ int ndbx_batch_set_coords(ndbx_ctx *ctx, const float *coords);
  * replays that completed, and replays rerun eagerly. */
 int ndbx_batch_info(ndbx_ctx *ctx, int *group_cells, int64_t *graph_replays,
                     int64_t *graph_reruns);
+/* Tile shape of the count: cells per counting group, and tiles per wave
+ * (1, or 4 when tiles of at most 16 queries share a wave; 0 while the
+ * choice is left to the tiles' mean size).  NDBX_AUTOTUNE picks both by
+ * timing the count on the context's input; the environment variables
+ * NDBX_GROUP_CELLS and NDBX_QUAD_TILES (0 or 1) fix them at creation.
+ * No counterpart in the reference (its count has no tiles). */
+int ndbx_tile_shape(ndbx_ctx *ctx, int *group_cells, int *tiles_per_wave);
 
 /* Largest input of ndbx_dbscan_noalloc, and the fixed device memory it
  * works in (held on each device the library runs on, until the process ends). */

@@ -69,6 +69,7 @@ static void open_common(Ctx &c, const float *coords, int64_t n_total, uint32_t f
     NDBX_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     c.prof.on = (flags & NDBX_PROFILE) != 0;
     if (const char *gc = std::getenv("NDBX_GROUP_CELLS")) c.group_cells = std::max(1, std::atoi(gc));
+    if (const char *qt = std::getenv("NDBX_QUAD_TILES")) c.quad_mode = std::atoi(qt) != 0;
     // fixed guard-band queue capacity of the MFMA count pass (tests of its overflow path)
     if (const char *bq = std::getenv("NDBX_BAND_QUEUE")) c.bq_fixed = std::max(1ll, std::atoll(bq));
     c.n = n_total;
@@ -129,8 +130,10 @@ static void run_label(Ctx &c) {
 }
 
 // Tile-shape autotuner: cells per counting group (the run of cells of one
-// row that share tiles) from {1, 2, 4, 8, 16}, chosen by the count kernel's
-// time on the context's own input.  Profiling data recorded so far is kept.
+// row that share tiles) from {1, 2, 4, 8, 16}, and tiles per wave (one, or
+// four tiles of at most 16 queries sharing a wave), chosen by the count
+// kernel's time on the context's own input.  Profiling data recorded so far
+// is kept.
 static void autotune(Ctx &c) {
     c.sync();
     Profiler saved_state = c.prof;
@@ -141,22 +144,26 @@ static void autotune(Ctx &c) {
     c.prof.calls.clear();
     c.prof.on = true;
     c.prof.only = "count";
-    int best = c.group_cells;
+    int best = c.group_cells, best_q = c.quad_mode;
     double best_ms = 1e300;
-    for (int gc : {1, 2, 4, 8, 16}) {
-        c.group_cells = gc;
-        run_classify(c);
-        c.sync();
-        const int slot = c.prof.slot_of("count");
-        const double ms = c.prof.calls[slot] ? c.prof.ms[slot] : 1e300;
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = gc;
+    for (int gc : {1, 2, 4, 8, 16})
+        for (int q : {0, 1}) {
+            c.group_cells = gc;
+            c.quad_mode = q;
+            run_classify(c);
+            c.sync();
+            const int slot = c.prof.slot_of("count");
+            const double ms = c.prof.calls[slot] ? c.prof.ms[slot] : 1e300;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = gc;
+                best_q = q;
+            }
+            c.prof.ms.assign(c.prof.ms.size(), 0.0);
+            c.prof.calls.assign(c.prof.calls.size(), 0);
         }
-        c.prof.ms.assign(c.prof.ms.size(), 0.0);
-        c.prof.calls.assign(c.prof.calls.size(), 0);
-    }
     c.group_cells = best;
+    c.quad_mode = best_q;
     c.tuned = true;
     // keep the event pool, restore what the caller had recorded
     saved_state.pool = c.prof.pool;
@@ -210,6 +217,8 @@ static void batch_run_graph(Ctx &c) {
     c.dcount.ensure(64);
     c.cap_cells = std::min<int64_t>(c.n, c.n_cells + c.n_cells / 4 + 1024);
     c.cap_tiles = std::min<int64_t>(c.n + c.cap_cells, c.n_tiles + c.n_tiles / 4 + 1024);
+    // a direct row table where the eager run took one
+    c.cap_rows = c.row_direct ? c.n_rows_direct + c.n_rows_direct / 4 + 1024 : 0;
     const bool prof_on = c.prof.on;
     try {
         // sizing run: grows every buffer to its capacity size, same results
@@ -400,6 +409,14 @@ int ndbx_batch_info(ndbx_ctx *h, int *group_cells, int64_t *graph_replays, int64
         if (group_cells) *group_cells = h->c.group_cells;
         if (graph_replays) *graph_replays = h->c.graph_replays;
         if (graph_reruns) *graph_reruns = h->c.graph_reruns;
+    });
+}
+
+int ndbx_tile_shape(ndbx_ctx *h, int *group_cells, int *tiles_per_wave) {
+    return guarded([&] {
+        need(h != nullptr, "ctx is null");
+        if (group_cells) *group_cells = h->c.group_cells;
+        if (tiles_per_wave) *tiles_per_wave = h->c.quad_mode < 0 ? 0 : h->c.quad_mode ? 4 : 1;
     });
 }
 

@@ -101,7 +101,7 @@ __global__ __launch_bounds__(64) void k_plan(const BoundsPart *__restrict__ part
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_inst) return;
     InstGrid g;
-    g.row_base = 0;  // captured runs use the row hash
+    g.row_base = 0;  // set by k_row_base where the run takes a direct row table
     const float e = eps[i];
     const float e2 = __fmul_rn(e, e);  // data_process.h:73
     g.e2 = e2;
@@ -160,6 +160,47 @@ __global__ __launch_bounds__(64) void k_plan(const BoundsPart *__restrict__ part
     if (fail) atomicOr(status, 2);
 }
 
+// Captured runs with a direct row table: each instance's first row (an
+// exclusive scan of G1 G2 over the instances, as grid_plan), in one block.
+// Rows beyond the table's capacity set bit 3 of *status and shrink every
+// instance to one cell, so that every later index stays in range (the run is
+// then done again eagerly).
+__global__ __launch_bounds__(1024) void k_row_base(InstGrid *__restrict__ ig, int n_inst, int64_t cap_rows,
+                                                   int32_t *__restrict__ status) {
+    __shared__ int64_t sh[1024];
+    __shared__ int64_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int b = 0; b < n_inst; b += 1024) {
+        const int i = b + (int)threadIdx.x;
+        const int64_t rows = i < n_inst ? ig[i].G[1] * ig[i].G[2] : 0;
+        sh[threadIdx.x] = rows;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            const int64_t v = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += v;
+            __syncthreads();
+        }
+        if (i < n_inst) ig[i].row_base = carry + sh[threadIdx.x] - rows;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += sh[1023];
+        __syncthreads();
+    }
+    if (carry > cap_rows) {
+        for (int i = (int)threadIdx.x; i < n_inst; i += 1024) {
+            InstGrid &g = ig[i];
+            for (int k = 0; k < 3; ++k) {
+                g.origin[k] = 0.0;
+                g.G[k] = 1;
+            }
+            g.inv_s = 1.0;
+            g.row_base = 0;
+        }
+        if (threadIdx.x == 0) atomicOr(status, 8);
+    }
+}
+
 // Decides the path and, for the grid, each instance's cell geometry.
 void grid_plan(Ctx &c) {
     if (c.capacity_mode) {
@@ -178,6 +219,9 @@ void grid_plan(Ctx &c) {
                 c.bounds_part.as<BoundsPart>(), d_first, reinterpret_cast<const float *>(d_first + c.n_inst + 1),
                 d_first + 2 * c.n_inst + 1, c.offsets_d.as<int64_t>(), c.n_inst, c.d, dg, c.lin_bits,
                 c.inst_d.as<InstGrid>(), c.dcount.as<int32_t>() + 2);
+            if (c.cap_rows > 0)
+                k_row_base<<<1, 1024, 0, c.stream>>>(c.inst_d.as<InstGrid>(), c.n_inst, c.cap_rows,
+                                                     c.dcount.as<int32_t>() + 2);
         }
         return;
     }
@@ -515,59 +559,125 @@ __global__ __launch_bounds__(256) void k_row_insert_direct(const uint64_t *__res
     if (last) table[row].y = (int32_t)(c + 1);
 }
 
+// The rows of cell c with one dz (plane dz + 2 of its 2k+1; one plane in 2-D
+// and 1-D): slots plane * P .. plane * P + P - 1.
 template <int D, bool DIRECT>
-__global__ __launch_bounds__(256) void k_stencil(const uint64_t *__restrict__ cell_key,
-                                                 const int32_t *__restrict__ cell_start, const int32_t *__restrict__ ncp,
-                                                 const InstGrid *__restrict__ ig, int lin_bits,
-                                                 const RowSlot *__restrict__ table, uint64_t mask,
-                                                 const int2 *__restrict__ dtable,
-                                                 int2 *__restrict__ ranges, const int32_t *__restrict__ cell_c) {
+__device__ __forceinline__ void stencil_plane(int64_t c, int plane, int64_t nc, const uint64_t *__restrict__ cell_key,
+                                              const InstGrid *__restrict__ ig, int lin_bits,
+                                              const RowSlot *__restrict__ table, uint64_t mask,
+                                              const int2 *__restrict__ dtable, int2 *__restrict__ ranges,
+                                              const int32_t *__restrict__ cell_c) {
     constexpr int R = kStencilRows<D>, k = 2;
-    const int64_t nc = *ncp;
-    // one thread per (cell, stencil row)
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= nc * R) return;
-    const int64_t c = tid / R;
-    const int slot = (int)(tid - c * R);
+    constexpr int P = D >= 2 ? 2 * k + 1 : 1;
     const uint64_t key = cell_key[c];
     const uint64_t inst = key >> lin_bits;
     const InstGrid &g = ig[inst];
     const int64_t G0 = g.G[0], G1 = g.G[1], G2 = g.G[2];
     const int64_t c0 = cell_c[c], c1 = cell_c[nc + c], c2 = cell_c[2 * nc + c];
-    const int dz = D >= 3 ? slot / (2 * k + 1) - k : 0;
-    const int dy = D >= 3 ? slot % (2 * k + 1) - k : (D == 2 ? slot - k : 0);
+    const int dz = D >= 3 ? plane - k : 0;
     const int64_t lo0 = c0 - k < 0 ? 0 : c0 - k;
     const int64_t hi0 = c0 + k >= G0 ? G0 - 1 : c0 + k;
-    int2 out = make_int2(0, 0);
-    const int64_t z = c2 + dz, y = c1 + dy;
-    int2 rv;
-    if (z >= 0 && z < G2 && y >= 0 && y < G1) {
-        const uint64_t rk = (inst << lin_bits) | (uint64_t)((z * G1 + y) * G0);
-        bool found;
-        if (DIRECT) {
-            rv = dtable[g.row_base + z * G1 + y];
-            found = rv.x < rv.y;
-        } else {
-            found = row_lookup(table, mask, rk, &rv);
-        }
-        if (found) {
-            int a = rv.x, b = rv.y;  // first cell with key >= rk+lo0
-            const uint64_t klo = rk + (uint64_t)lo0, khi = rk + (uint64_t)hi0;
-            while (a < b) {
-                int mid = (a + b) >> 1;
-                if (cell_key[mid] < klo) a = mid + 1;
-                else b = mid;
+    const int64_t z = c2 + dz;
+#pragma unroll
+    for (int r = 0; r < P; ++r) {
+        const int dy = D >= 2 ? r - k : 0;
+        const int64_t y = c1 + dy;
+        int2 out = make_int2(0, 0);
+        if (z >= 0 && z < G2 && y >= 0 && y < G1) {
+            const uint64_t rk = (inst << lin_bits) | (uint64_t)((z * G1 + y) * G0);
+            int2 rv;
+            bool found;
+            if (DIRECT) {
+                rv = dtable[g.row_base + z * G1 + y];
+                found = rv.x < rv.y;
+            } else {
+                found = row_lookup(table, mask, rk, &rv);
             }
-            int e = a, f = rv.y;
-            while (e < f) {  // first cell with key > khi
-                int mid = (e + f) >> 1;
-                if (cell_key[mid] <= khi) e = mid + 1;
-                else f = mid;
+            if (found) {
+                int a = rv.x, b = rv.y;  // first cell with key >= rk+lo0
+                const uint64_t klo = rk + (uint64_t)lo0, khi = rk + (uint64_t)hi0;
+                while (a < b) {
+                    int mid = (a + b) >> 1;
+                    if (cell_key[mid] < klo) a = mid + 1;
+                    else b = mid;
+                }
+                int e = a, f = rv.y;
+                while (e < f) {  // first cell with key > khi
+                    int mid = (e + f) >> 1;
+                    if (cell_key[mid] <= khi) e = mid + 1;
+                    else f = mid;
+                }
+                out = make_int2(a, e);
             }
-            out = make_int2(a, e);
         }
+        ranges[c * R + plane * P + r] = out;
     }
-    ranges[c * R + slot] = out;
+}
+
+// One thread per (cell, plane) for planes [plane_lo, plane_lo + n_planes):
+// a plane's rows are consecutive in the direct table and in the ranges.
+template <int D, bool DIRECT>
+__global__ __launch_bounds__(256) void k_stencil(const uint64_t *__restrict__ cell_key, const int32_t *__restrict__ ncp,
+                                                 const InstGrid *__restrict__ ig, int lin_bits,
+                                                 const RowSlot *__restrict__ table, uint64_t mask,
+                                                 const int2 *__restrict__ dtable, int2 *__restrict__ ranges,
+                                                 const int32_t *__restrict__ cell_c, int plane_lo, int n_planes) {
+    const int64_t nc = *ncp;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= nc * n_planes) return;
+    const int64_t c = tid / n_planes;
+    const int plane = plane_lo + (int)(tid - c * n_planes);
+    stencil_plane<D, DIRECT>(c, plane, nc, cell_key, ig, lin_bits, table, mask, dtable, ranges, cell_c);
+}
+
+// The planes below the centre (dz < 0) of the cells the border search reads
+// them for: the first and last cell of each listed tile and the cell of each
+// listed point.  A cell met more than once is written with the same values.
+template <int D, bool DIRECT>
+__global__ __launch_bounds__(256) void k_stencil_back(const int4 *__restrict__ ctiles,
+                                                      const int32_t *__restrict__ blist,
+                                                      const int32_t *__restrict__ n_blist,
+                                                      const int32_t *__restrict__ plist,
+                                                      const int32_t *__restrict__ n_plist,
+                                                      const int32_t *__restrict__ cell_of,
+                                                      const uint64_t *__restrict__ cell_key,
+                                                      const int32_t *__restrict__ ncp, const InstGrid *__restrict__ ig,
+                                                      int lin_bits, const RowSlot *__restrict__ table, uint64_t mask,
+                                                      const int2 *__restrict__ dtable, int2 *__restrict__ ranges,
+                                                      const int32_t *__restrict__ cell_c) {
+    const int64_t nc = *ncp, nbt = *n_blist, npt = *n_plist;
+    const int64_t total = 4 * nbt + 2 * npt;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t cell;
+        if (i < 4 * nbt) {
+            const int4 tile = ctiles[blist[i >> 2]];
+            cell = (i & 2) ? tile.y : tile.x;
+        } else {
+            cell = cell_of[plist[(i - 4 * nbt) >> 1]];
+        }
+        if (cell < nc) stencil_plane<D, DIRECT>(cell, (int)(i & 1), nc, cell_key, ig, lin_bits, table, mask, dtable,
+                                                ranges, cell_c);
+    }
+}
+
+void stencil_back(Ctx &c, const int32_t *blist, const int32_t *n_blist, const int32_t *plist,
+                  const int32_t *n_plist) {
+    if (!c.back_rows_pending) return;
+    c.back_rows_pending = false;
+    const hipStream_t s = c.stream;
+    const int32_t *ncp = c.dcount.as<int32_t>();
+    const InstGrid *ig = c.inst_d.as<InstGrid>();
+    Launch L(c.prof, s, "stencil");
+    if (c.row_direct)
+        k_stencil_back<3, true><<<2048, 256, 0, s>>>(c.ctiles.as<int4>(), blist, n_blist, plist, n_plist,
+                                                     c.cell_of.as<int32_t>(), c.cell_key.as<uint64_t>(), ncp, ig,
+                                                     c.lin_bits, nullptr, 0, c.row_keys.as<int2>(),
+                                                     c.ranges.as<int2>(), c.cell_c.as<int32_t>());
+    else
+        k_stencil_back<3, false><<<2048, 256, 0, s>>>(c.ctiles.as<int4>(), blist, n_blist, plist, n_plist,
+                                                      c.cell_of.as<int32_t>(), c.cell_key.as<uint64_t>(), ncp, ig,
+                                                      c.lin_bits, c.row_keys.as<RowSlot>(), c.row_mask, nullptr,
+                                                      c.ranges.as<int2>(), c.cell_c.as<int32_t>());
 }
 
 // Counting groups: the cells of one row whose c0 share c0 / group_cells; the
@@ -1072,12 +1182,12 @@ __device__ __forceinline__ void count_tile_sym(int64_t t, const int4 *__restrict
     const int my_beg = cell_start[my_cell], my_end = cell_start[my_cell + 1];
     const InstGrid &g = ig[cell_key[cf] >> lin_bits];
     const float e2 = g.e2;
-    int2 w = tile_window<D>(lane, rows_per_cell, cf, cl, lo, hi, g, cell_c[nc + cf], cell_c[2 * nc + cf], ranges,
-                            cell_c, nc);
-    // rows before the centre row hold lower cells only; the centre row keeps
-    // the cells after the tile's last cell
+    // rows before the centre row hold lower cells only (and are not read:
+    // 3-D grids compute them for the border search only); the centre row
+    // keeps the cells after the tile's last cell
     const int centre = rows_per_cell >> 1;
-    if (lane < centre) w = make_int2(0, 0);
+    int2 w = tile_window<D>(lane < centre ? rows_per_cell : lane, rows_per_cell, cf, cl, lo, hi, g, cell_c[nc + cf],
+                            cell_c[2 * nc + cf], ranges, cell_c, nc);
     if (lane == centre && w.x < cl + 1) w.x = cl + 1;
     if (w.x >= w.y) w = make_int2(0, 0);
     int ps = w.x < w.y ? cell_start[w.x] : 0, pe = w.x < w.y ? cell_start[w.y] : 0;
@@ -1690,7 +1800,7 @@ static void grid_classify_d(Ctx &c) {
     const int ncb = (int)ceil_div(nc, 256);
     // part table (before the dense re-sort: ordinary cells keep this order)
     const int32_t *part_off = nullptr;
-    if (c.sub_bits > 0) {
+    if (D == 2 && c.sub_bits > 0) {  // read by the dense 2-D count only
         c.part_off.ensure((size_t)nc * 8 * 4);
         Launch L(c.prof, s, "part_off");
         k_part_off<D><<<(unsigned)ceil_div(nc << D, 256), 256, 0, s>>>(keys, c.sub_bits, c.cell_start.as<int32_t>(),
@@ -1743,10 +1853,12 @@ static void grid_classify_d(Ctx &c) {
     }
     // eager runs take a direct row table when it is no larger than the hash
     // (8-byte entries, at most 2 per hash slot); captured runs the hash
-    const bool direct = !cap_mode && c.n_rows_direct <= 2 * (int64_t)cap;
+    // (captured runs: when the eager run took one, sized by cap_rows)
+    const bool direct = cap_mode ? c.cap_rows > 0 : c.n_rows_direct <= 2 * (int64_t)cap;
     if (direct) {
-        c.row_keys.ensure((size_t)c.n_rows_direct * sizeof(int2));
-        fill_u64(c.row_keys.as<uint64_t>(), 0, c.n_rows_direct, s, c.prof);  // (0, 0): empty row
+        const int64_t n_rows = cap_mode ? c.cap_rows : c.n_rows_direct;
+        c.row_keys.ensure((size_t)n_rows * sizeof(int2));
+        fill_u64(c.row_keys.as<uint64_t>(), 0, n_rows, s, c.prof);  // (0, 0): empty row
         Launch L(c.prof, s, "row_insert");
         k_row_insert_direct<<<ncb, 256, 0, s>>>(c.cell_key.as<uint64_t>(), ncp, ig, c.lin_bits,
                                                 c.cell_c.as<int32_t>(), c.row_keys.as<int2>());
@@ -1759,16 +1871,24 @@ static void grid_classify_d(Ctx &c) {
     }
     const int rows = c.rows_per_cell = kStencilRows<D>;
     c.ranges.ensure((size_t)nc * rows * 8);
+    // 3-D: the planes from the centre on (dz >= 0) here, the ones below
+    // for the cells the border search reads them for (stencil_back); the
+    // count and the union read rows from the centre row on only
+    c.back_rows_pending = D == 3;
+    c.row_direct = direct;
+    c.row_mask = cap - 1;
     {
+        const int plane_lo = D == 3 ? 2 : 0, n_planes = D == 3 ? 3 : 1;
+        const unsigned nbs = (unsigned)ceil_div(nc * n_planes, 256);
         Launch L(c.prof, s, "stencil");
         if (direct)
-            k_stencil<D, true><<<(unsigned)ceil_div(nc * rows, 256), 256, 0, s>>>(
-                c.cell_key.as<uint64_t>(), c.cell_start.as<int32_t>(), ncp, ig, c.lin_bits, nullptr, 0,
-                c.row_keys.as<int2>(), c.ranges.as<int2>(), c.cell_c.as<int32_t>());
+            k_stencil<D, true><<<nbs, 256, 0, s>>>(c.cell_key.as<uint64_t>(), ncp, ig, c.lin_bits, nullptr, 0,
+                                                  c.row_keys.as<int2>(), c.ranges.as<int2>(), c.cell_c.as<int32_t>(),
+                                                  plane_lo, n_planes);
         else
-            k_stencil<D, false><<<(unsigned)ceil_div(nc * rows, 256), 256, 0, s>>>(
-                c.cell_key.as<uint64_t>(), c.cell_start.as<int32_t>(), ncp, ig, c.lin_bits, c.row_keys.as<RowSlot>(),
-                cap - 1, nullptr, c.ranges.as<int2>(), c.cell_c.as<int32_t>());
+            k_stencil<D, false><<<nbs, 256, 0, s>>>(c.cell_key.as<uint64_t>(), ncp, ig, c.lin_bits,
+                                                   c.row_keys.as<RowSlot>(), cap - 1, nullptr, c.ranges.as<int2>(),
+                                                   c.cell_c.as<int32_t>(), plane_lo, n_planes);
     }
     // count tiles over groups of cells of one row
     c.gflag.ensure(nc * 4);
@@ -1827,7 +1947,7 @@ static void grid_classify_d(Ctx &c) {
         // small tiles share a wave (4-instance batch of 50M 3-D points: count
         // 21.2 -> 18.4 ms); on denser grids the second launch only costs
         // (10M 3-D: 11.4 -> 11.9 ms), so it is left out
-        const int quads = c.n_tiles > 0 && c.n < 8 * c.n_tiles;
+        const int quads = c.quad_mode >= 0 ? c.quad_mode : c.n_tiles > 0 && c.n < 8 * c.n_tiles;
         k_count_sym<D><<<(unsigned)ceil_div(nct, 4), 256, 0, s>>>(
             c.ctiles.as<int4>(), ntp, c.cell_start.as<int32_t>(), c.cell_of.as<int32_t>(), c.cell_key.as<uint64_t>(),
             ig, c.lin_bits, c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), ncp, c.soa.as<float>(), npad,

@@ -169,8 +169,12 @@ struct Ctx {
         misc, scratch, scratch2, soa_core, cell_c, gflag, ctile_off, ctiles, tile_tests, dense_id, dense_info, dense_tmp, fine_off, part_off, part_fine, big_id;
     int64_t n_cells = 0, n_tiles = 0, n_label_tiles = 0, pair_tests = 0, n_dense = 0;
     int64_t n_rows_direct = 0;  // rows of all instances' (c2, c1) grids (eager runs)
+    bool row_direct = false;          // the last grid's row table: direct (else hashed, row_mask)
+    uint64_t row_mask = 0;
+    bool back_rows_pending = false;   // 3-D stencil planes below the centre not yet computed
     int lin_bits = 0, cell_bits = 0, sub_bits = 0, key_bits = 0, reach_max = 2, rows_per_cell = 1;
     int group_cells = 8;  // cells of one row per counting group
+    int quad_mode = -1;   // small tiles four per wave: 1 yes, 0 no, -1 by the tiles' mean size
     bool tuned = false;   // group_cells chosen by the autotuner
     std::vector<InstGrid> h_inst;
 
@@ -179,6 +183,7 @@ struct Ctx {
     // the capacities) and is captured into a graph that later runs replay
     bool capacity_mode = false;
     int64_t cap_cells = 0, cap_tiles = 0, n_chunks = 0;
+    int64_t cap_rows = 0;  // captured runs: direct row table entries (0: the row hash)
     DevBuf dcount;   // int32: cells, tiles, status (bit 1: a count over capacity, bit 2: plan outside layout)
     DevBuf plan_in;  // chunk table, first chunk per instance, eps, min_pts
     hipGraph_t graph = nullptr;
@@ -214,6 +219,11 @@ struct Ctx {
 void grid_plan(Ctx &c);            // bounds -> path choice and per-instance grids
 void grid_classify(Ctx &c);        // keys, sort, cells, stencil, counts
 void grid_label(Ctx &c);           // union-find over cells, border, labels
+// 3-D: the stencil planes below the centre for the cells the border search
+// reads (the first and last cell of each listed tile, the cell of each listed
+// point); lists and their lengths on the device
+void stencil_back(Ctx &c, const int32_t *blist, const int32_t *n_blist, const int32_t *plist,
+                  const int32_t *n_plist);
 void brute_classify(Ctx &c);
 void brute_label(Ctx &c);
 void mfma_classify(Ctx &c);        // d >= 4: partition, bf16 MFMA filter, counts

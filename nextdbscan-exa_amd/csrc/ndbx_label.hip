@@ -1031,6 +1031,7 @@ static void grid_label_d(Ctx &c) {
                                         n, c.labels.as<int32_t>());
         k_border_list<<<(unsigned)ceil_div(std::max<int64_t>(nct, 1), kBorderListThreads), kBorderListThreads, 0, s>>>(
             c.ctiles.as<int4>(), ntp, c.core_s.as<uint8_t>(), c.nn_s.as<int32_t>(), blist, n_blist, plist, n_plist);
+        stencil_back(c, blist, n_blist, plist, n_plist);
         k_border_pts<D><<<nb, 256, 0, s>>>(plist, n_plist, c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), ig,
                                            c.lin_bits, c.ranges.as<int2>(), rows, c.cell_c.as<int32_t>(), ncp,
                                            c.cell_of.as<int32_t>(), c.soa.as<float>(), c.soa_core.as<float>(), npad,

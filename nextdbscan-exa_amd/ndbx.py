@@ -62,6 +62,7 @@ SIGNATURES = {
     "ndbx_batch_meta": ([_P, _P], ctypes.c_int),
     "ndbx_batch_set_coords": ([_P, _P], ctypes.c_int),
     "ndbx_batch_info": ([_P, _IP, _I64P, _I64P], ctypes.c_int),
+    "ndbx_tile_shape": ([_P, _IP, _IP], ctypes.c_int),
     "ndbx_set_chunking": ([_P, _I64], ctypes.c_int),
     "ndbx_chunk_info": ([_P, _I64P], ctypes.c_int),
     "ndbx_dbscan_noalloc": ([_P, _I64, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_uint32, _P, _P, _P,
@@ -265,6 +266,13 @@ class Batch:
         rr = ctypes.c_int64()
         _check(lib().ndbx_batch_info(self._h, ctypes.byref(gc), ctypes.byref(rp), ctypes.byref(rr)))
         return gc.value, rp.value, rr.value
+
+    def tile_shape(self):
+        """(cells per counting group, tiles per wave: 1, 4, or 0 while left to the tiles' mean size)."""
+        gc = ctypes.c_int()
+        tw = ctypes.c_int()
+        _check(lib().ndbx_tile_shape(self._h, ctypes.byref(gc), ctypes.byref(tw)))
+        return gc.value, tw.value
 
     def nn(self):
         out = np.empty(self.n, np.int32)

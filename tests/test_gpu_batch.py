@@ -87,3 +87,22 @@ def test_batch_tiny_instances(gpu_lib, d):
         np.testing.assert_array_equal(nn[s:e], nn_o, err_msg=f"instance {i}")
         np.testing.assert_array_equal(lab[s:e], lab_o, err_msg=f"instance {i}")
         assert tuple(meta[i]) == meta_o, i
+
+
+@pytest.mark.parametrize("quad", ["0", "1"])
+def test_batch_tiles_per_wave_against_oracle(gpu_lib, monkeypatch, quad):
+    """Small 3-D tiles counted one per wave and four per wave (16 lanes
+    each) give the oracle's counts and labels, instance by instance."""
+    x, off, eps, m = make_batch(24, 3, 11, max_n=1500)
+    monkeypatch.setenv("NDBX_QUAD_TILES", quad)
+    print(f"case batch tiles per wave quad={quad} n={len(x)}", flush=True)
+    with gpu_lib.Batch(x, off, eps, m) as b:
+        b.run()
+        assert b.tile_shape()[1] == (4 if quad == "1" else 1)
+        nn, lab, meta = b.nn(), b.labels(), b.meta()
+    for i in range(len(off) - 1):
+        lo, hi = int(off[i]), int(off[i + 1])
+        nn_o, lab_o, meta_o = O.dbscan(x[lo:hi], eps[i], int(m[i]))
+        np.testing.assert_array_equal(nn[lo:hi], nn_o)
+        np.testing.assert_array_equal(lab[lo:hi], lab_o)
+        assert tuple(meta[i]) == meta_o

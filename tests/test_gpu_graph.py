@@ -101,6 +101,7 @@ def test_autotune_keeps_results(gpu_lib):
         b.run()
         gc, _, _ = b.info()
         assert gc in (1, 2, 4, 8, 16)
+        assert b.tile_shape() in [(gc, 1), (gc, 4)]
         check_same((b.nn(), b.labels(), b.meta()), want, "tuned")
         b.run()
         check_same((b.nn(), b.labels(), b.meta()), want, "tuned replay")
