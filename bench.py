@@ -340,7 +340,7 @@ def bench_single(workload, steps, warmup, n=None, cpu=True, compare_plain=False,
     return out
 
 
-def bench_batch(n_inst, steps, warmup, cpu=True, threads=16):
+def bench_batch(n_inst, steps, warmup, cpu=True, threads=16, other_forms=True):
     import numpy as np
     import torch
     import ndbx
@@ -364,6 +364,8 @@ def bench_batch(n_inst, steps, warmup, cpu=True, threads=16):
     out["roofline"] = roofline_for(r["dominant"], r["avg_s"], n, d, stats, sum_nn,
                                    f"batch3d_{n_inst}") if r["avg_s"] else None
     out["kernels"] = dict(list(r["kernels"].items())[:8])
+    if not other_forms:  # the eager batched run alone (kernel profiles)
+        return out
     # the batch replayed from a captured graph, tile shape from the autotuner
     with ndbx.Batch(x, off, eps_arr, mins, flags=ndbx.GRAPH | ndbx.AUTOTUNE) as gb:
         for _ in range(max(2, warmup)):
@@ -441,7 +443,9 @@ def main():
     ap.add_argument("--forms", default="blobs2d_100k,clusters64d_1m,batch3d,skew2d_100m",
                     help="comma-separated list of the other workloads to measure")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-plain", action="store_true", help="skip the all-pairs comparison of the MFMA workload")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the all-pairs comparison of the MFMA workload, and the graph and per-instance forms "
+                         "of the batch")
     ap.add_argument("--no-audit", action="store_true", help="skip the float64 audit of the MFMA filter")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run one device-to-device copy of BYTES (a known-size kernel for "
@@ -473,7 +477,7 @@ def main():
 
     progress(f"headline {args.workload}")
     if args.workload.startswith("batch3d"):
-        h = bench_batch(args.n_inst, args.steps, args.warmup, cpu=cpu, threads=threads)
+        h = bench_batch(args.n_inst, args.steps, args.warmup, cpu=cpu, threads=threads, other_forms=not args.no_plain)
         ms_per_step = h["eager"]["ms_per_step"]
     else:
         h = bench_single(args.workload, args.steps, args.warmup, n=args.n, cpu=cpu, threads=threads,
