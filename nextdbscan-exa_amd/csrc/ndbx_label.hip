@@ -867,76 +867,66 @@ __global__ __launch_bounds__(256, 8) void k_border(const int4 *__restrict__ ctil
 __global__ __launch_bounds__(256) void k_meta_points(const uint8_t *__restrict__ core, const int32_t *__restrict__ labels,
                                                      const int64_t *__restrict__ off, int n_inst, int64_t n,
                                                      int32_t *__restrict__ meta) {
+    // each wave takes one contiguous span of points, 64 at a time; the
+    // instance of the span's first point is searched once, later ones are
+    // reached by stepping over instance boundaries
     const int lane = threadIdx.x & 63;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    int a_acc = 0, b_acc = 0;  // single-instance accumulators
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n; base += stride) {
-        const int64_t i = base + lane;
-        const bool v = i < n;
-        const int c = v && core[i];
-        const int z = v && labels[i] == NDBX_NO_CLUSTER;
-        if (n_inst == 1) {
-            a_acc += c;
-            b_acc += z;
-            continue;
+    const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const int64_t wave = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int64_t span = (n + n_waves * 64 - 1) / (n_waves * 64) * 64;
+    const int64_t s0 = wave * span, s1 = s0 + span < n ? s0 + span : n;
+    if (s0 >= s1) return;
+    int inst = 0;
+    {
+        int lo = 0, hi = n_inst - 1;  // last instance starting at or before s0
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= s0) lo = mid;
+            else hi = mid - 1;
         }
-        // the instance of the wave's first point (a wave-uniform search), and
-        // of each lane only when the wave crosses into the next instance
-        int first = 0;
-        {
-            int lo = 0, hi = n_inst - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (off[mid] <= base) lo = mid;
-                else hi = mid - 1;
-            }
-            first = lo;
-        }
-        const int64_t last = base + 63 < n ? base + 63 : n - 1;
-        int inst = first;
-        if (off[first + 1] <= last && v) {
-            int lo = first, hi = n_inst - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (off[mid] <= i) lo = mid;
-                else hi = mid - 1;
-            }
-            inst = lo;
-        }
-        if (off[first + 1] > last) {
-            int a = c, b = z;
-            for (int o = 32; o >= 1; o >>= 1) {
-                a += __shfl_xor(a, o, 64);
-                b += __shfl_xor(b, o, 64);
-            }
-            if (lane == 0) {
-                if (a) atomicAdd(meta + 3 * first + 0, a);
-                if (b) atomicAdd(meta + 3 * first + 1, b);
-            }
-        } else {
-            if (c) atomicAdd(meta + 3 * inst + 0, 1);
-            if (z) atomicAdd(meta + 3 * inst + 1, 1);
-        }
+        inst = lo;
     }
-    if (n_inst == 1) {
-        __shared__ int sa[4], sb[4];
+    int64_t next = off[inst + 1];  // start of the next instance
+    int a_acc = 0, b_acc = 0;      // counts of instance `inst` so far
+    auto flush = [&]() {
         for (int o = 32; o >= 1; o >>= 1) {
             a_acc += __shfl_xor(a_acc, o, 64);
             b_acc += __shfl_xor(b_acc, o, 64);
         }
         if (lane == 0) {
-            sa[threadIdx.x >> 6] = a_acc;
-            sb[threadIdx.x >> 6] = b_acc;
+            if (a_acc) atomicAdd(meta + 3 * inst + 0, a_acc);
+            if (b_acc) atomicAdd(meta + 3 * inst + 1, b_acc);
         }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const int a = sa[0] + sa[1] + sa[2] + sa[3], b = sb[0] + sb[1] + sb[2] + sb[3];
-            if (a) atomicAdd(meta + 0, a);
-            if (b) atomicAdd(meta + 1, b);
+        a_acc = b_acc = 0;
+    };
+    for (int64_t base = s0; base < s1; base += 64) {
+        const int64_t i = base + lane;
+        const bool v = i < s1;
+        const int c = v && core[i];
+        const int z = v && labels[i] == NDBX_NO_CLUSTER;
+        while (next <= base) {  // the wave has left instance `inst`
+            flush();
+            ++inst;
+            next = off[inst + 1];
+        }
+        const int64_t last = base + 63 < s1 ? base + 63 : s1 - 1;
+        if (next > last) {
+            a_acc += c;
+            b_acc += z;
+        } else if (v) {  // the 64 points cross into later instances: per lane
+            int li = inst;
+            while (off[li + 1] <= i) ++li;
+            if (li == inst) {
+                a_acc += c;
+                b_acc += z;
+            } else {
+                if (c) atomicAdd(meta + 3 * li + 0, 1);
+                if (z) atomicAdd(meta + 3 * li + 1, 1);
+            }
         }
     }
+    flush();
 }
-
 __global__ __launch_bounds__(256) void k_meta_cells(const int32_t *__restrict__ cell_has_core,
                                                     const uint64_t *__restrict__ cell_key, int lin_bits,
                                                     const int32_t *__restrict__ parent, const int32_t *__restrict__ ncp,
