@@ -377,7 +377,14 @@ __global__ __launch_bounds__(256) void k_keys(const float *__restrict__ x, int64
                                               uint32_t *__restrict__ vals) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int inst = n_inst > 1 ? find_inst(off, n_inst, i) : 0;
+    // the instance of the wave's first point (one wave-uniform search), then
+    // per lane only where the wave's points cross into later instances
+    int inst = 0;
+    if (n_inst > 1) {
+        const int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+        inst = find_inst(off, n_inst, base);
+        while (off[inst + 1] <= i) ++inst;
+    }
     const InstGrid &g = ig[inst];
     uint64_t lin = 0;
     uint32_t sub = 0;
