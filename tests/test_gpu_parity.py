@@ -188,3 +188,37 @@ def test_dense_cube_parts_against_oracle(gpu_lib, kind):
     np.testing.assert_array_equal(nn, nn_o)
     np.testing.assert_array_equal(lab, lab_o)
     assert meta == meta_o
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_border_cores_below_and_above(gpu_lib, seed):
+    """3-D border points whose only cores lie in the stencil planes below
+    their cell (dz < 0: computed only for the border search) or above it,
+    in tiles of a few points and in tiles of many, between two dense balls;
+    labels must be the oracle's (smallest adjacent cluster id)."""
+    rng = np.random.default_rng(40 + seed)
+    eps, m = np.float32(1.0), 150
+    balls = [np.array(c, np.float32) for c in [(5, 5, 5), (5, 5, 7.6), (9, 5, 5)]]
+    parts = [c + rng.normal(0, 0.2, (300, 3)).astype(np.float32) for c in balls]
+    # thin shells just above / below the balls: too sparse to hold cores
+    for c in balls:
+        for dz in (1.0, -1.0):
+            sh = rng.normal(0, 0.35, (25, 3)).astype(np.float32)
+            sh[:, 2] = c[2] + dz + rng.uniform(-0.1, 0.1, 25).astype(np.float32)
+            sh[:, :2] += c[:2]
+            parts.append(sh)
+    # a dense-ish slab of non-core points between the first two balls
+    slab = np.stack([rng.uniform(4.0, 6.0, 60), rng.uniform(4.0, 6.0, 60), rng.uniform(6.2, 6.4, 60)], 1)
+    parts.append(slab.astype(np.float32))
+    parts.append(rng.uniform(0, 12, (200, 3)).astype(np.float32))
+    x = np.ascontiguousarray(np.concatenate(parts, 0))
+    print(f"case border below/above seed={seed} n={len(x)}", flush=True)
+    _, nn, lab, meta, _, path = run(gpu_lib, x, eps, m)
+    nn_o, lab_o, meta_o = O.dbscan(x, eps, m)
+    assert path == "grid"
+    np.testing.assert_array_equal(nn, nn_o)
+    np.testing.assert_array_equal(lab, lab_o)
+    assert meta == meta_o
+    core = nn_o >= m
+    border = (~core) & (lab_o != np.iinfo(np.int32).max)
+    assert border.sum() > 50 and core.sum() > 500
