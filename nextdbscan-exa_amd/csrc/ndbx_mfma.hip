@@ -788,14 +788,15 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
     bf16x8_t ahi[KS];
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) ahi[ks] = arow_ok ? hi8[arow * (KP / 8) + ks * 2 + half] : bf16x8_t{};
-    // accumulator rows: r -> tile row 8 (r/4) + 4 half + (r%4); rows past the
-    // tile have mid = +inf, so dv = -inf: never counted, never in the band
-    float mid_r[16];
+    // accumulator rows: r -> tile row 8 (r/4) + 4 half + (r%4); the start
+    // value of every block is -(|a|^2 - e2)/2 per row (-inf for rows past
+    // the tile, so dv = -inf: never counted, never in the band)
+    f32x16_t acc0;
     int32_t rcnt[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int tr = 8 * (r >> 2) + 4 * half + (r & 3);
-        mid_r[r] = tr < nrows ? 0.5f * (A.na[row0 + tr] - A.e2) : INF;
+        acc0[r] = tr < nrows ? -(0.5f * (A.na[row0 + tr] - A.e2)) : -INF;
         rcnt[r] = 0;
     }
     float rmax_row = arow_ok ? A.rho[arow] : 0.0f;
@@ -897,21 +898,25 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
                 hb = (float)(A.b1 * R * R + A.b2 * R + A.b3);  // half the band
                 // skip threshold of the hi*hi partial sum: the two products
                 // it leaves out are each at most 2^-8 (1 + 2^-8)^2 |a| |b|
-                // (|lo| <= 2^-8 |x| componentwise), 0.008 R^2 together; its
+                // (|lo| <= 2^-8 |x| componentwise), 0.00787 R^2 together; its
                 // own float32 accumulation error and the full sum's are each
-                // within hb, so a partial below -(3 hb + 0.008 R^2) leaves the
-                // full dv below -hb
+                // within hb, so a partial below -(3 hb + 0.00787 R^2) leaves
+                // the full dv below -hb.  The 0.008 R^2 taken also covers the
+                // one rounding of the per-column threshold below, at most
+                // 2^-24 (|hskip| + R^2 / 2).
                 hskip = -(float)(3.0 * (double)hb + 0.008 * R * R);
             }
-            // the accumulator starts at -|b|^2/2 - (|a|^2 - e2)/2 and ends at
-            // dv = (e2 - est)/2 (-inf for rows and columns out of range)
+            // dv = (e2 - est)/2 = -(|a|^2 - e2)/2 + a.b - |b|^2/2 (-inf for
+            // rows and columns out of range).  The hi*hi partial is taken
+            // without the column term -|b|^2/2, which moves to the skip
+            // threshold instead (+inf for columns out of range), and is added
+            // only in the blocks that go on to the full products.
             const float nbh = -0.5f * sb_na[buf][col];
-            f32x16_t acc;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[r] = nbh - mid_r[r];
+            const float hcol = hskip - nbh;
             // hi*hi first: a block whose partial largest dv is below hskip
             // holds no neighbour and no band element under the full
             // products, and costs 2 MFMAs per 16-wide k step less
+            f32x16_t acc = acc0;
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
                 const bf16x8_t bhi = sb_hi[buf][col * ROWU + ks * 2 + half];
@@ -921,7 +926,9 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
 #pragma unroll
             for (int r = 1; r < 16; ++r) hmax = fmaxf(hmax, acc[r]);
             int32_t ccnt = 0;
-            if (__ballot(hmax >= hskip) != 0) {
+            if (__ballot(hmax >= hcol) != 0) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[r] += nbh;
             const int64_t bpt = bcb + col < n_last ? bcb + col : n_last;  // clamped as in fetch
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
@@ -1267,10 +1274,11 @@ static MfmaArgs mfma_args(Ctx &c) {
     const int kp = kp_for(c.d);
     const double u = std::ldexp(1.0, -24);
     // the last terms cover the count pass: its accumulator starts at
-    // -|b|^2/2 - (|a|^2 - e2)/2 (the threshold folded in) and every value it
-    // takes is at most |a||b| + (|a|^2 + |b|^2)/2 + e2/2 <= 2 R^2 + e2/2 in
-    // magnitude: one rounding of that per MFMA (3 kp / 16 of them) and four
-    // more for the start value and (|a|^2 - e2)/2
+    // -(|a|^2 - e2)/2 (the threshold folded in), takes -|b|^2/2 after the
+    // hi*hi products, and every value it takes is at most
+    // |a||b| + (|a|^2 + |b|^2)/2 + e2/2 <= 2 R^2 + e2/2 in magnitude: one
+    // rounding of that per MFMA (3 kp / 16 of them) and four more for
+    // (|a|^2 - e2)/2, |b|^2/2 and the addition of the column term
     const double cnt_r = (3.0 * kp / 16.0 + 4.0) * u;
     a.b1 = 2.0 * (3.1 * std::ldexp(1.0, -16) + 3.0 * kp * u) + (2.0 * kp + 16.0) * u + 2.0 * cnt_r;
     a.b2 = std::ldexp(1.0, -21) * std::sqrt((double)a.e2 * 1.01);
