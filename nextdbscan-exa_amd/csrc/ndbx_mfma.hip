@@ -37,6 +37,33 @@ __device__ __forceinline__ T ld_const(const T *p) {
 }
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
+// LDS reads of 16 bytes, waited for in the same statement: one at p, or
+// four at p + k * STRIDE bytes.  Used where the LDS is filled by loads
+// straight from global memory (global_load_lds): the compiler cannot tell
+// which buffer such a load writes and would otherwise wait for every load in
+// flight before each read; the caller orders the fills with explicit waits
+// and barriers instead.
+__device__ __forceinline__ uint32_t lds_addr(const void *p) {
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void *)p;
+}
+__device__ __forceinline__ bf16x8_t lds_read16(const bf16x8_t *p) {
+    bf16x8_t v;
+    asm volatile("ds_read_b128 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(v) : "v"(lds_addr(p)));
+    return v;
+}
+template <int STRIDE>
+__device__ __forceinline__ void lds_read16x4(const bf16x8_t *p, bf16x8_t &v0, bf16x8_t &v1, bf16x8_t &v2,
+                                             bf16x8_t &v3) {
+    asm volatile(
+        "ds_read_b128 %0, %4\n\t"
+        "ds_read_b128 %1, %4 offset:%5\n\t"
+        "ds_read_b128 %2, %4 offset:%6\n\t"
+        "ds_read_b128 %3, %4 offset:%7\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(v0), "=&v"(v1), "=&v"(v2), "=&v"(v3)
+        : "v"(lds_addr(p)), "i"(STRIDE), "i"(2 * STRIDE), "i"(3 * STRIDE));
+}
+
 constexpr int kCellTarget = 1024;  // points per partition cell, on average
 
 // ---------------------------------------------------------------- prep --
@@ -757,15 +784,16 @@ constexpr int kCountOcc = NDBX_MF_COUNT_OCC;
 
 template <int KP, int WAVES>
 __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 2) void k_mf_count(MfmaArgs A, int chunk) {
+    static_assert(WAVES == 4, "a block's loads are split over four waves");
     constexpr int KS = KP / 16;
     constexpr int UNITS = 32 * (KP / 8);  // bf16x8 units of a block, per hi / lo
-    constexpr int ROWU = KP / 8 + 1;
-    constexpr int NT = 64 * WAVES;
-    constexpr int UPT = (UNITS + NT - 1) / NT;  // hi units per thread
-    // only the hi halves are staged: the lo halves are read from global
-    // memory by the few blocks that need the full products
-    __shared__ bf16x8_t sb_hi[2][32 * ROWU];
-    __shared__ float sb_na[2][32];
+    // Only the hi halves are staged (the lo halves are read from global
+    // memory by the few blocks that need the full products), unit-major:
+    // unit u of column c at u * 32 + c, so the 32 columns a wave's half
+    // reads for one k step are 32 consecutive slots.  Three buffers: the
+    // block in the products and the next two, loaded straight into LDS.
+    __shared__ bf16x8_t sb_hi[3][UNITS];
+    __shared__ float sb_na[3][32];
     __shared__ int32_t sb_cacc[kColWin];  // column counts of the current window
     __shared__ int32_t sb_edge;
     __shared__ int4 sb_q[WAVES][kBandQ];  // per wave: guard-band pairs not yet in the global queue
@@ -825,71 +853,90 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
     };
     // block sequence: the 32-column blocks of the cells this cell takes
     const int c_beg = ld_const(A.cand_off + cell), c_end = ld_const(A.cand_off + cell + 1);
-    int ci = c_beg, cb = 0, cs = 0, ce = 0, co = -1;
-    auto settle = [&]() {  // first block at or after (ci, cb) of a taken cell
-        while (ci < c_end) {
-            if (co >= 0 && cb < ce) return;
-            if (co >= 0) ++ci;
-            co = -1;
-            if (ci >= c_end) return;
+    // cursor: cell co (candidate ci), columns [cb, cb + 32) of [cs, ce)
+    int ci = c_beg - 1, cb = 0, cs = 0, ce = 0, co = -1;
+    auto next_cell = [&]() {  // the next taken cell with points (ci = c_end: none)
+        for (++ci; ci < c_end; ++ci) {
             const int o = ld_const(A.cand + ci);
-            if (!mf_takes(cell, o)) {
-                ++ci;
-                continue;
-            }
-            co = o;
-            cs = cb = ld_const(A.cell_start + o);
-            ce = ld_const(A.cell_start + o + 1);
+            if (!mf_takes(cell, o)) continue;
+            const int s0 = ld_const(A.cell_start + o), s1 = ld_const(A.cell_start + o + 1);
+            if (s0 >= s1) continue;
+            co = o, cs = cb = s0, ce = s1;
+            return;
         }
     };
-    // Columns past the cell end keep whatever the clamped loads return (the
-    // rows of the next cell, or the last row): their norm is +inf, so their
-    // accumulator is -inf whatever the products are.
-    // (The norm is selected only when published: a select right after its
-    // load would wait for every load of the fetch.)
-    bf16x8_t pf[UPT];
-    float pf_na = INF;
-    bool pf_ok = false;
+    auto advance = [&]() {
+        cb += 32;
+        if (cb >= ce) next_cell();
+    };
+    // A block's loads go straight to LDS (no registers, so two blocks can be
+    // in flight behind the one in the products): each wave loads KP of its
+    // UNITS hi units, 64 per instruction, and 8 of its 32 norms, so every
+    // wave issues kLoads loads per block and waiting until at most kLoads are
+    // outstanding leaves the previous block landed.  Rows are read 32 at a
+    // time from the block's first (from row n past the sequence's end): the
+    // hi rows and the norms hold 32 rows of padding.  Columns past their
+    // cell's end are given norm +inf (accumulator -inf) where the norm is
+    // read.
+    constexpr int kOps = (KP + 63) / 64;
+    constexpr int kLoads = kOps + 1;
+    static_assert(kLoads < 16, "vmcnt field");
     const int64_t n_last = A.n - 1;
-    auto fetch = [&]() {
+    // per-lane byte offsets of the loads within a block
+    uint32_t hoff[kOps];
 #pragma unroll
-        for (int q = 0; q < UPT; ++q) {
-            const int u = threadIdx.x + q * NT;
-            const int bc = u / (KP / 8), kk = u - bc * (KP / 8);
-            const int64_t pt = cb + bc < n_last ? cb + bc : n_last;
-            if (u < UNITS) pf[q] = hi8[pt * (KP / 8) + kk];
-        }
-        const int jn = cb + (int)(threadIdx.x & 31);
-        pf_na = A.na[jn < n_last ? jn : n_last];
-        pf_ok = jn < ce;
-    };
-    auto publish = [&](int buf) {
+    for (int q = 0; q < kOps; ++q) {
+        const int slot = wave * KP + 64 * q + lane;
+        hoff[q] = (uint32_t)((slot & 31) * (KP * 2) + (slot >> 5) * 16);
+    }
+    const uint32_t noff = (uint32_t)(8 * wave + lane) * 4;
+    auto fetch = [&](int b) {
+        const int fb = cb < (int)A.n ? cb : (int)A.n;  // past the sequence's end: any rows in range
+        const char *hb8 = reinterpret_cast<const char *>(hi8) + (int64_t)fb * (KP * 2);
+        const char *nb8 = reinterpret_cast<const char *>(A.na + fb);
 #pragma unroll
-        for (int q = 0; q < UPT; ++q) {
-            const int u = threadIdx.x + q * NT;
-            const int bc = u / (KP / 8), kk = u - bc * (KP / 8);
-            if (u < UNITS) sb_hi[buf][bc * ROWU + kk] = pf[q];
+        for (int q = 0; q < kOps; ++q) {
+            if (lane < KP - 64 * q)
+                __builtin_amdgcn_global_load_lds((const void *)(hb8 + hoff[q]),
+                                                 (__attribute__((address_space(3))) void *)&sb_hi[b][wave * KP + 64 * q],
+                                                 16, 0, 0);
         }
-        if (threadIdx.x < 32) sb_na[buf][threadIdx.x] = pf_ok ? pf_na : INF;
+        if (lane < 8)
+            __builtin_amdgcn_global_load_lds((const void *)(nb8 + noff),
+                                             (__attribute__((address_space(3))) void *)&sb_na[b][8 * wave], 4, 0, 0);
     };
-    settle();
+    // every wave's loads of the older block landed and its LDS operations
+    // done, then the workgroup barrier; the empty asm statements keep the
+    // compiler from moving memory operations across it
+    auto landed_barrier = [&]() {
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_waitcnt(0x0070 | kLoads);  // vmcnt(kLoads) lgkmcnt(0)
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+    };
+    next_cell();
     if (ci >= c_end) return;  // cannot happen: a cell always takes itself
     for (int t = threadIdx.x; t < kColWin; t += blockDim.x) sb_cacc[t] = 0;
     if (threadIdx.x == 0) sb_edge = 0;
-    fetch();
-    publish(0);
-    __syncthreads();
+    fetch(0);
+    // the block in the products (cell o0, columns [cb0, ce0)) and, at the
+    // cursor, the next one (m1: there is one)
+    int o0 = co, cb0 = cb, ci0 = ci, cs0 = cs, ce0 = ce;
+    advance();
+    bool m1 = ci < c_end;
+    fetch(1);
+    landed_barrier();
     int buf = 0, cur_o = -1;
     float hb = 0.0f, hskip = 0.0f;
     for (;;) {
-        // this block's place in the sequence (wave-uniform, in scalar
-        // registers), then the next block's loads, which overlap this
-        // block's products
-        const int o = co, bcb = cb, bci = ci, bcs = cs;
-        cb += 32;
-        settle();
-        const bool more = ci < c_end;
-        if (more) fetch();
+        // this block's place in the sequence and the next one's
+        // (wave-uniform, in scalar registers), then the loads of the block
+        // after that, which overlap this block's products
+        const int o = o0, bcb = cb0, bci = ci0, bcs = cs0, bce = ce0;
+        const int o1 = co, cb1 = cb, ci1 = ci, cs1 = cs, ce1 = ce;
+        advance();
+        const bool m2 = ci < c_end;
+        fetch(buf == 0 ? 2 : buf - 1);
         const bool diag = o == cell;
         if (wave_on) {
             if (o != cur_o) {
@@ -911,16 +958,26 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
             // without the column term -|b|^2/2, which moves to the skip
             // threshold instead (+inf for columns out of range), and is added
             // only in the blocks that go on to the full products.
-            const float nbh = -0.5f * sb_na[buf][col];
+            const float nav = sb_na[buf][col];
+            const float nbh = bcb + col < bce ? -0.5f * nav : -INF;
             const float hcol = hskip - nbh;
             // hi*hi first: a block whose partial largest dv is below hskip
             // holds no neighbour and no band element under the full
             // products, and costs 2 MFMAs per 16-wide k step less
             f32x16_t acc = acc0;
+            const bf16x8_t *bp = &sb_hi[buf][half * 32 + col];
 #pragma unroll
-            for (int ks = 0; ks < KS; ++ks) {
-                const bf16x8_t bhi = sb_hi[buf][col * ROWU + ks * 2 + half];
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[ks], bhi, acc, 0, 0, 0);
+            for (int k4 = 0; k4 < KS; k4 += 4) {
+                // B operands of four k steps (slots 64 apart), read together
+                bf16x8_t b4[4];
+                if (KS - k4 >= 4) {
+                    lds_read16x4<64 * 16>(bp + k4 * 64, b4[0], b4[1], b4[2], b4[3]);
+                } else {
+                    for (int k = 0; k < KS - k4; ++k) b4[k] = lds_read16(bp + (k4 + k) * 64);
+                }
+#pragma unroll
+                for (int k = 0; k < 4 && k4 + k < KS; ++k)
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[k4 + k], b4[k], acc, 0, 0, 0);
             }
             float hmax = acc[0];
 #pragma unroll
@@ -929,12 +986,12 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
             if (__ballot(hmax >= hcol) != 0) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[r] += nbh;
-            const int64_t bpt = bcb + col < n_last ? bcb + col : n_last;  // clamped as in fetch
+            const int64_t bpt = bcb + col < n_last ? bcb + col : n_last;  // clamped to the last row
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
                 const bf16x8_t blo = lo8[bpt * (KP / 8) + ks * 2 + half];
                 const bf16x8_t alo = arow_ok ? lo8[arow * (KP / 8) + ks * 2 + half] : bf16x8_t{};
-                const bf16x8_t bhi = sb_hi[buf][col * ROWU + ks * 2 + half];
+                const bf16x8_t bhi = lds_read16(&sb_hi[buf][(ks * 2 + half) * 32 + col]);
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[ks], blo, acc, 0, 0, 0);
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi, acc, 0, 0, 0);
             }
@@ -1005,13 +1062,12 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
             }
             }
         }
-        if (more) publish(buf ^ 1);
-        __syncthreads();
+        landed_barrier();
         // leaving the window (another cell, the next kColWin columns, or the
         // end): its column counts and pair flag go to global memory
-        if (!more || co != o || (cb - cs) / kColWin != (bcb - bcs) / kColWin) {
+        if (!m1 || o1 != o || (uint32_t)(cb1 - cs1) / kColWin != (uint32_t)(bcb - bcs) / kColWin) {
             if (!diag) {
-                const int32_t wb = bcs + (bcb - bcs) / kColWin * kColWin;
+                const int32_t wb = bcs + (int32_t)((uint32_t)(bcb - bcs) / kColWin * kColWin);
                 for (int t = threadIdx.x; t < kColWin; t += blockDim.x) {
                     const int32_t v = sb_cacc[t];
                     if (v != 0) {
@@ -1026,9 +1082,13 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
             }
             __syncthreads();
         }
-        if (!more) break;
-        buf ^= 1;
+        if (!m1) break;
+        o0 = o1, cb0 = cb1, ci0 = ci1, cs0 = cs1, ce0 = ce1;
+        m1 = m2;
+        buf = __builtin_amdgcn_readfirstlane(buf == 2 ? 0 : buf + 1);
     }
+    // no load into LDS outlives the workgroup
+    __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) lgkmcnt(0)
     if (!wave_on) return;
     band_flush();
     // each row over the 32 lanes (columns) of its half
@@ -1315,7 +1375,9 @@ void mfma_classify(Ctx &c) {
         NDBX_HIP(hipMemcpyAsync(cent + j * d, c.coords + (j * n / k_cent) * d, d * 4, hipMemcpyDeviceToDevice, s));
     c.keys.ensure(n * 8);
     c.vals.ensure(n * 4);
-    c.keys2.ensure(std::max<size_t>(n * 8, (size_t)n * kp * 2));
+    // the hi rows are read 32 at a time from any row up to n: 32 rows of
+    // padding past the last
+    c.keys2.ensure(std::max<size_t>(n * 8, (size_t)(n + 32) * kp * 2));
     c.vals2.ensure(std::max<size_t>(n * 4, (size_t)n * kp * 2));
     {
         Launch L(c.prof, s, "mf_assign");
@@ -1334,7 +1396,7 @@ void mfma_classify(Ctx &c) {
     if (alt) {
         std::swap(c.keys, c.keys2);
         std::swap(c.vals, c.vals2);
-        c.keys2.ensure((size_t)n * kp * 2);
+        c.keys2.ensure((size_t)(n + 32) * kp * 2);
         c.vals2.ensure((size_t)n * kp * 2);
     }
     const uint32_t *perm = c.vals.as<uint32_t>();
@@ -1351,13 +1413,17 @@ void mfma_classify(Ctx &c) {
     NDBX_HIP(hipMemcpyAsync(&h_total, d_total, 4, hipMemcpyDeviceToHost, s));
     // sorted rows, hi/lo, norms
     c.soa.ensure((size_t)n * d * 4);
-    c.soa_core.ensure((size_t)n * 2 * 4);
+    c.soa_core.ensure(((size_t)n * 2 + 32) * 4);  // norms, radii, 32 floats read past them
     {
         Launch L(c.prof, s, "mf_prep");
         k_mfma_prep<<<nb, 256, 0, s>>>(c.coords, perm, n, d, kp, mu, c.soa.as<float>(), c.keys2.as<__bf16>(),
                                        reinterpret_cast<__bf16 *>(c.vals2.p), c.soa_core.as<float>(),
                                        c.soa_core.as<float>() + n);
     }
+    // the padding rows hold zeros (their columns are out of range wherever
+    // they are read)
+    fill_u64(reinterpret_cast<uint64_t *>(c.keys2.as<__bf16>() + (size_t)n * kp), 0, 8 * kp, s, c.prof);
+    fill_u64(reinterpret_cast<uint64_t *>(c.soa_core.as<float>() + 2 * n), 0, 16, s, c.prof);
     c.sync();
     const int64_t nc = h_total;
     c.n_cells = nc;
