@@ -19,6 +19,14 @@
 //    est <= e2 - band are neighbours, est > e2 + band are not, and the rest
 //    are decided by the reference's own float32 arithmetic on the original
 //    coordinates.  Every decision therefore equals the float32 predicate.
+// 4. Block pruning: within a cell, points are ordered by their distance to
+//    the cell's centroid m_C, so a 32-point tile's radius (largest distance)
+//    is small except for the cell's outliers.  For every tile t and cell C,
+//    T[t][C] bounds from below the distance of t's points to m_C; a row group
+//    G of cell A and a column tile J of cell B hold no pair within r when
+//    max(T[G][B] - rad(J), T[J][A] - rad(G)) > r (triangle inequality), and
+//    only neighbours when rad(G) + |m_A - m_B| + rad(J) is small enough; the
+//    count pass computes neither kind (the second is counted wholesale).
 #include <cfloat>
 #include <climits>
 
@@ -65,6 +73,9 @@ __device__ __forceinline__ void lds_read16x4(const bf16x8_t *p, bf16x8_t &v0, bf
 }
 
 constexpr int kCellTarget = 1024;  // points per partition cell, on average
+// largest size of the block pruning's distance bounds (two tiles x cells
+// float tables); past it the count pass computes every candidate block
+constexpr size_t kPruneBytes = (size_t)16 << 30;
 
 // ---------------------------------------------------------------- prep --
 __global__ __launch_bounds__(256) void k_colsum(const float *__restrict__ x, int64_t n, int d,
@@ -121,29 +132,57 @@ __global__ __launch_bounds__(256) void k_assign(const float *__restrict__ x, int
 
 __device__ __forceinline__ __bf16 to_bf16(float f) { return (__bf16)f; }  // round to nearest even
 
-// Sorted float32 rows, centred bf16 hi/lo rows padded to kp, |a'|^2 in
-// float32 (ascending k) and |a'|.
+// Sorted float32 rows, centred bf16 hi/lo rows padded to KP, |a'|^2 in
+// float32 (ascending k) and |a'|.  One thread per row, held in registers:
+// the bf16 rows are written 16 bytes at a time, and with vec (d % 4 == 0
+// and x 16-byte aligned) so are the float32 rows.
+template <int KP>
 __global__ __launch_bounds__(256) void k_mfma_prep(const float *__restrict__ x, const uint32_t *__restrict__ perm,
-                                                   int64_t n, int d, int kp, const float *__restrict__ mu,
+                                                   int64_t n, int d, bool vec, const float *__restrict__ mu,
                                                    float *__restrict__ xs, __bf16 *__restrict__ hi,
                                                    __bf16 *__restrict__ lo, float *__restrict__ na,
                                                    float *__restrict__ rho) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int64_t p = perm[i];
-    float s = 0.0f;
-    for (int k = 0; k < kp; ++k) {
-        float a = 0.0f;
-        if (k < d) {
-            const float v = x[p * d + k];
-            xs[i * d + k] = v;
-            a = v - mu[k];
+    float v[KP];
+    if (vec) {
+        const float4 *src = reinterpret_cast<const float4 *>(x + p * d);
+        float4 *dst = reinterpret_cast<float4 *>(xs + i * d);
+#pragma unroll
+        for (int k4 = 0; k4 < KP / 4; ++k4) {
+            float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (4 * k4 < d) {
+                q = src[k4];
+                dst[k4] = q;
+            }
+            v[4 * k4] = q.x, v[4 * k4 + 1] = q.y, v[4 * k4 + 2] = q.z, v[4 * k4 + 3] = q.w;
         }
+    } else {
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            v[k] = 0.0f;
+            if (k < d) {
+                v[k] = x[p * d + k];
+                xs[i * d + k] = v[k];
+            }
+        }
+    }
+    float s = 0.0f;
+    bf16x8_t h8[KP / 8], l8[KP / 8];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+        const float a = k < d ? v[k] - mu[k] : 0.0f;
         const __bf16 h = to_bf16(a);
-        const __bf16 l = to_bf16(a - (float)h);
-        hi[i * kp + k] = h;
-        lo[i * kp + k] = l;
+        h8[k / 8][k % 8] = h;
+        l8[k / 8][k % 8] = to_bf16(a - (float)h);
         s = __fadd_rn(s, __fmul_rn(a, a));
+    }
+    bf16x8_t *hd = reinterpret_cast<bf16x8_t *>(hi + i * KP), *ld = reinterpret_cast<bf16x8_t *>(lo + i * KP);
+#pragma unroll
+    for (int u = 0; u < KP / 8; ++u) {
+        hd[u] = h8[u];
+        ld[u] = l8[u];
     }
     na[i] = s;
     rho[i] = sqrtf(s) * (1.0f + 1e-6f);
@@ -187,6 +226,240 @@ __global__ __launch_bounds__(256) void k_cell_radius(const float *__restrict__ x
         rad[c] = sqrt(sr[0]) * (1.0 + 1e-12);
         rmax[c] = sm[0];
     }
+}
+
+// Each unordered pair of candidate cells is computed once: cell c takes
+// candidate o when o == c, or when (c < o) equals (c + o even) -- about half
+// of every cell's candidates, whatever the cell order.  A pair (c, o != c)
+// adds its decisions to the rows' counts and to the columns' counts; the
+// diagonal pair (c, c) adds to the rows only (every row of c meets every
+// column of c, itself included).
+__device__ __forceinline__ bool mf_takes(int c, int o) { return o == c || ((c < o) == (((c + o) & 1) == 0)); }
+
+// ------------------------------------------------------ block pruning --
+// Per cell: the centroid of its points (float64 sums, rounded to float32);
+// the float32 centroid is the reference point of every radius and distance
+// bound of the block pruning.
+__global__ __launch_bounds__(128) void k_cell_centroid(const float *__restrict__ x, const uint32_t *__restrict__ perm,
+                                                       const int32_t *__restrict__ cell_start, int64_t nc, int d,
+                                                       float *__restrict__ mc) {
+    const int64_t c = blockIdx.x;
+    if (c >= nc) return;
+    const int32_t b = cell_start[c], e = cell_start[c + 1];
+    for (int k = threadIdx.x; k < d; k += blockDim.x) {
+        double s = 0.0;
+        for (int32_t i = b; i < e; ++i) s += (double)x[(int64_t)perm[i] * d + k];
+        mc[c * d + k] = e > b ? (float)(s / (double)(e - b)) : 0.0f;
+    }
+}
+
+// Sort keys of the order within cells: (centre index, distance to the cell
+// centroid as float bits, which order like the distances).
+__global__ __launch_bounds__(256) void k_radius_keys(const float *__restrict__ x, const uint32_t *__restrict__ perm,
+                                                     const uint64_t *__restrict__ keys,
+                                                     const int32_t *__restrict__ cell_of,
+                                                     const float *__restrict__ mc, int64_t n, int d,
+                                                     uint64_t *__restrict__ keys2, uint32_t *__restrict__ vals2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t p = perm[i];
+    const float *m = mc + (int64_t)cell_of[i] * d;
+    float s = 0.0f;
+    for (int k = 0; k < d; ++k) {
+        const float df = x[(int64_t)p * d + k] - m[k];
+        s += df * df;
+    }
+    keys2[i] = keys[i] << 32 | (uint64_t)__float_as_uint(sqrtf(s));
+    vals2[i] = p;
+}
+
+// Per sorted point: an upper bound of its distance to its cell's centroid
+// (float64, rounded up to float32).
+__global__ __launch_bounds__(256) void k_point_radius(const float *__restrict__ xs, const int32_t *__restrict__ cell_of,
+                                                      const float *__restrict__ mc, int64_t n, int d,
+                                                      float *__restrict__ rad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *m = mc + (int64_t)cell_of[i] * d;
+    double s = 0.0;
+    for (int k = 0; k < d; ++k) {
+        const double df = (double)xs[i * d + k] - (double)m[k];
+        s += df * df;
+    }
+    const double r = sqrt(s) * (1.0 + 1e-12);
+    float f = (float)r;
+    if ((double)f < r) f = __uint_as_float(__float_as_uint(f) + 1u);  // the next float up (f >= 0, finite)
+    rad[i] = f;
+}
+
+// Per row tile t (up to 32 sorted points of one cell) and cell C: tab[t][C],
+// a lower bound of the smallest distance of t's points to C's centroid, and
+// rad_t[t], the largest radius of t's points.  A float32 fused sum of d
+// squared rounded differences is within (d + 1) 2^-24 (relative, first
+// order) of the exact square distance, so s (1 - 2^-15) is below it for
+// d <= 128.
+template <int KP>
+__global__ __launch_bounds__(256) void k_tile_dist(const float *__restrict__ xs, const int4 *__restrict__ tiles,
+                                                   const float *__restrict__ mc, int64_t nc, int d,
+                                                   const float *__restrict__ rad, float *__restrict__ tab,
+                                                   float *__restrict__ rad_t) {
+    __shared__ float sp[32 * KP];
+    const int64_t t = blockIdx.x;
+    const int4 tile = tiles[t];
+    const int rows = tile.z;
+    for (int u = threadIdx.x; u < 32 * KP; u += blockDim.x) {
+        const int r = u / KP, k = u - r * KP;
+        sp[u] = (r < rows && k < d) ? xs[(int64_t)(tile.y + r) * d + k] : 0.0f;
+    }
+    if (threadIdx.x < 64) {
+        float m = threadIdx.x < rows ? rad[tile.y + threadIdx.x] : 0.0f;
+        for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        if (threadIdx.x == 0) rad_t[t] = m;
+    }
+    __syncthreads();
+    for (int64_t C = threadIdx.x; C < nc; C += blockDim.x) {
+        float m[KP];
+#pragma unroll
+        for (int k = 0; k < KP; ++k) m[k] = k < d ? mc[C * d + k] : 0.0f;
+        float best = __builtin_inff();
+        for (int r = 0; r < rows; ++r) {
+            float s = 0.0f;
+#pragma unroll
+            for (int k = 0; k < KP; ++k) {
+                const float df = sp[r * KP + k] - m[k];
+                s = __fmaf_rn(df, df, s);
+            }
+            best = fminf(best, s);
+        }
+        tab[t * nc + C] = sqrtf(best * (1.0f - 3.0517578125e-05f)) * (1.0f - 2.0f * 5.9604644775390625e-08f);
+    }
+}
+
+// tab (tiles x cells) to cell-major (cells x tiles), 32 x 32 tiles via LDS.
+__global__ __launch_bounds__(256) void k_transpose_f32(const float *__restrict__ in, int64_t rows, int64_t cols,
+                                                       float *__restrict__ out) {
+    __shared__ float tl[32][33];
+    const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int y = ty; y < 32; y += 8)
+        if (r0 + y < rows && c0 + tx < cols) tl[y][tx] = in[(r0 + y) * cols + c0 + tx];
+    __syncthreads();
+    for (int y = ty; y < 32; y += 8)
+        if (c0 + y < cols && r0 + tx < rows) out[(c0 + y) * rows + r0 + tx] = tl[tx][y];
+}
+
+// Per count group g (row tiles [grp.y, grp.y + grp.z) of cell A) and
+// candidate entry i of A that A takes (candidate cell B), column tile j of B
+// (j < 64) is
+//  - pruned when no point of it is within r of a row of the group:
+//    max(T[G][B] - rad(j), T[j][A] - rad(G)) > r, with T[G][B] the smallest
+//    of the group's tiles' bounds and rad(G) their largest radius;
+//  - wholly within eps when every pair is a neighbour: the pair distance is
+//    at most U = rad(G) + |m_A - m_B| + rad(j), and the reference's float32
+//    sum is within (d + 3) 2^-24 of the exact square distance, so
+//    U^2 (1 + 1.01 (d + 3) 2^-24) <= e2 makes the predicate accept them all.
+//    Such a tile is counted here: each row of the group gains its columns,
+//    each of its columns the group's rows (not in A's own pair, where the
+//    rows' counts take every pair), through per-tile sums (tacc) that
+//    k_tile_acc adds to every point of the tile; the pair is flagged for the
+//    label passes;
+//  - otherwise computed by the count pass: bit j of masks[moff[g] + i].
+// Entries A does not take are 0.  Adds the point pairs of the computed
+// tiles (rows x columns) to *pairs.  tcm: the bounds cell-major; mc: the
+// centroids.
+__global__ __launch_bounds__(256) void k_block_masks(const int4 *__restrict__ groups, const int32_t *__restrict__ cand_off,
+                                                     const int32_t *__restrict__ cand,
+                                                     const int32_t *__restrict__ toff, const int4 *__restrict__ tiles,
+                                                     const float *__restrict__ tcm, const float *__restrict__ rad_t,
+                                                     int64_t nt, const float *__restrict__ mc, int d, double r,
+                                                     double e2_in, const int32_t *__restrict__ moff,
+                                                     uint64_t *__restrict__ masks, int32_t *__restrict__ tacc,
+                                                     uint8_t *__restrict__ pflag,
+                                                     unsigned long long *__restrict__ pairs) {
+    __shared__ int32_t s_rowadd;
+    const int64_t g = blockIdx.x;
+    const int4 grp = groups[g];
+    const int A = grp.x;
+    if (threadIdx.x == 0) s_rowadd = 0;
+    float rg = 0.0f;
+    int rows = 0;
+    for (int u = 0; u < grp.z; ++u) {
+        rg = fmaxf(rg, rad_t[grp.y + u]);
+        rows += tiles[grp.y + u].z;
+    }
+    __syncthreads();
+    const int32_t cb = cand_off[A], ce = cand_off[A + 1];
+    const float *tA = tcm + (int64_t)A * nt;  // tiles to A's centroid
+    const float *mA = mc + (int64_t)A * d;
+    unsigned long long kept = 0;
+    int32_t rowadd = 0;
+    for (int32_t i = cb + threadIdx.x; i < ce; i += blockDim.x) {
+        const int B = cand[i];
+        uint64_t m = 0;
+        if (mf_takes(A, B)) {
+            const float *tB = tcm + (int64_t)B * nt;  // tiles to B's centroid
+            float tg = __builtin_inff();
+            for (int u = 0; u < grp.z; ++u) tg = fminf(tg, tB[grp.y + u]);
+            double dd = 0.0;
+            const float *mB = mc + (int64_t)B * d;
+            for (int k = 0; k < d; ++k) {
+                const double df = (double)mA[k] - (double)mB[k];
+                dd += df * df;
+            }
+            const double dab = sqrt(dd) * (1.0 + 1e-12);  // above |m_A - m_B|
+            const int32_t j0 = toff[B], j1 = toff[B + 1];
+            const int nj = j1 - j0 < 64 ? j1 - j0 : 64;
+            bool any_in = false;
+            for (int j = 0; j < nj; ++j) {
+                const double rj = (double)rad_t[j0 + j];
+                const double lb = fmax((double)tg - rj, (double)tA[j0 + j] - (double)rg);
+                if (lb > r) continue;  // pruned
+                const double ub = (double)rg + dab + rj;
+                const int cols = tiles[j0 + j].z;
+                if (ub * ub * (1.0 + 1.01 * (d + 3) * 5.9604644775390625e-08) * (1.0 + 1e-12) <= e2_in) {
+                    any_in = true;
+                    rowadd += cols;
+                    if (B != A) atomicAdd(tacc + j0 + j, rows);
+                } else {
+                    m |= 1ull << j;
+                    kept += (unsigned long long)cols;
+                }
+            }
+            for (int j = j0 + 64; j < j1; ++j) kept += (unsigned long long)tiles[j].z;
+            if (any_in) pflag[i] = 1;
+        }
+        masks[moff[g] + (i - cb)] = m;
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        kept += __shfl_xor(kept, o, 64);
+        rowadd += __shfl_xor(rowadd, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (kept) atomicAdd(pairs, kept * (unsigned long long)rows);
+        if (rowadd) atomicAdd(&s_rowadd, rowadd);
+    }
+    __syncthreads();
+    if (threadIdx.x < grp.z && s_rowadd) atomicAdd(tacc + grp.y + threadIdx.x, s_rowadd);
+}
+
+// Per sorted point: its row tile's sum from k_block_masks added to its count.
+__global__ __launch_bounds__(256) void k_tile_acc(const int32_t *__restrict__ tacc, const int32_t *__restrict__ cell_of,
+                                                  const int32_t *__restrict__ cell_start,
+                                                  const int32_t *__restrict__ toff, int64_t n,
+                                                  int32_t *__restrict__ nn_s) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = cell_of[i];
+    const int32_t v = tacc[toff[c] + (int32_t)((i - cell_start[c]) >> 5)];
+    if (v) nn_s[i] += v;
+}
+
+// Per count group: its cell's candidate entries (mask words).
+__global__ __launch_bounds__(256) void k_group_entries(const int4 *__restrict__ groups, int64_t ng,
+                                                       const int32_t *__restrict__ cand_off,
+                                                       int32_t *__restrict__ cnt) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < ng) cnt[g] = cand_off[groups[g].x + 1] - cand_off[groups[g].x];
 }
 
 // Candidate cell pairs: |centre_C - centre_C'| <= R_C + R_C' + r (+ slack),
@@ -323,6 +596,8 @@ struct MfmaArgs {
     int64_t n_groups;
     const int32_t *rows;  // tile row -> point (null: identity)
     const int32_t *cand_off, *cand, *cell_start;
+    const uint64_t *bmask;     // count pass: kept column tiles per (group, candidate entry), or null
+    const int32_t *bmask_off;  // per group: its first mask word
     const __bf16 *hi, *lo;
     const float *na, *rho, *cell_rmax, *xs;
     int64_t n;
@@ -364,13 +639,6 @@ __global__ __launch_bounds__(256) void k_group_count(const int32_t *__restrict__
     if (c < nc) cnt[c] = (toff[c + 1] - toff[c] + per - 1) / per;
 }
 
-// Each unordered pair of candidate cells is computed once: cell c takes
-// candidate o when o == c, or when (c < o) equals (c + o even) -- about half
-// of every cell's candidates, whatever the cell order.  A pair (c, o != c)
-// adds its decisions to the rows' counts and to the columns' counts; the
-// diagonal pair (c, c) adds to the rows only (every row of c meets every
-// column of c, itself included).
-__device__ __forceinline__ bool mf_takes(int c, int o) { return o == c || ((c < o) == (((c + o) & 1) == 0)); }
 
 // PASS 0: neighbour counts.  PASS 1: unite core pairs, over the cell pairs
 // this cell takes whose count pass found a pair within eps (col > row in
@@ -851,22 +1119,36 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
         }
         qn = 0;
     };
-    // block sequence: the 32-column blocks of the cells this cell takes
+    // block sequence: the 32-column blocks of the cells this cell takes,
+    // the first 64 of a cell those its mask keeps (all without masks)
     const int c_beg = ld_const(A.cand_off + cell), c_end = ld_const(A.cand_off + cell + 1);
-    // cursor: cell co (candidate ci), columns [cb, cb + 32) of [cs, ce)
+    const uint64_t *gmask = A.bmask ? A.bmask + (ld_const(A.bmask_off + gidx) - c_beg) : nullptr;
+    // cursor: cell co (candidate ci), columns [cb, cb + 32) of [cs, ce);
+    // bm: the current cell's kept tiles among its first 64 not yet visited
     int ci = c_beg - 1, cb = 0, cs = 0, ce = 0, co = -1;
-    auto next_cell = [&]() {  // the next taken cell with points (ci = c_end: none)
+    uint64_t bm = 0;
+    auto next_cell = [&]() {  // the next taken cell with blocks to compute (ci = c_end: none)
         for (++ci; ci < c_end; ++ci) {
             const int o = ld_const(A.cand + ci);
             if (!mf_takes(cell, o)) continue;
             const int s0 = ld_const(A.cell_start + o), s1 = ld_const(A.cell_start + o + 1);
             if (s0 >= s1) continue;
-            co = o, cs = cb = s0, ce = s1;
+            const uint64_t m = gmask ? ld_const(gmask + ci) : ~0ull;
+            if (m == 0 && s1 - s0 <= 64 * 32) continue;  // every block pruned
+            co = o, cs = s0, ce = s1;
+            bm = m;
+            cb = bm ? s0 + 32 * __builtin_ctzll(bm) : s0 + 64 * 32;
+            if (cb >= ce) continue;
             return;
         }
     };
     auto advance = [&]() {
-        cb += 32;
+        if (cb < cs + 64 * 32) {
+            bm &= bm - 1;
+            cb = bm ? cs + 32 * __builtin_ctzll(bm) : cs + 64 * 32;
+        } else {
+            cb += 32;
+        }
         if (cb >= ce) next_cell();
     };
     // A block's loads go straight to LDS (no registers, so two blocks can be
@@ -1411,21 +1693,48 @@ void mfma_classify(Ctx &c) {
                c.cell_start.as<int32_t>(), c.cell_key.as<uint64_t>(), d_total, c.scratch, s, c.prof);
     int32_t h_total = 0;
     NDBX_HIP(hipMemcpyAsync(&h_total, d_total, 4, hipMemcpyDeviceToHost, s));
+    c.sync();
+    const int64_t nc = h_total;
+    // within each cell, points in ascending distance to the cell's centroid
+    // (the block pruning's tiles: small radii except for a cell's outliers)
+    c.mfp_cent.ensure((size_t)nc * d * 4);
+    {
+        Launch L(c.prof, s, "mf_prune");
+        k_cell_centroid<<<(unsigned)nc, 128, 0, s>>>(c.coords, perm, c.cell_start.as<int32_t>(), nc, d,
+                                                    c.mfp_cent.as<float>());
+        k_radius_keys<<<nb, 256, 0, s>>>(c.coords, perm, c.keys.as<uint64_t>(), c.cell_of.as<int32_t>(),
+                                         c.mfp_cent.as<float>(), n, d, c.keys2.as<uint64_t>(), c.vals2.as<uint32_t>());
+    }
+    bool alt2 = false;
+    radix_sort_u64(c.keys2.as<uint64_t>(), c.vals2.as<uint32_t>(), c.keys.as<uint64_t>(), c.vals.as<uint32_t>(), n,
+                   bits + 32, c.scratch, s, c.prof, &alt2);
+    if (!alt2) {  // the order lands in keys / vals, the hi / lo rows in keys2 / vals2
+        std::swap(c.keys, c.keys2);
+        std::swap(c.vals, c.vals2);
+        c.keys2.ensure(std::max<size_t>(n * 8, (size_t)(n + 32) * kp * 2));
+        c.vals2.ensure(std::max<size_t>(n * 4, (size_t)n * kp * 2));
+    }
+    perm = c.vals.as<uint32_t>();
     // sorted rows, hi/lo, norms
     c.soa.ensure((size_t)n * d * 4);
     c.soa_core.ensure(((size_t)n * 2 + 32) * 4);  // norms, radii, 32 floats read past them
     {
         Launch L(c.prof, s, "mf_prep");
-        k_mfma_prep<<<nb, 256, 0, s>>>(c.coords, perm, n, d, kp, mu, c.soa.as<float>(), c.keys2.as<__bf16>(),
-                                       reinterpret_cast<__bf16 *>(c.vals2.p), c.soa_core.as<float>(),
-                                       c.soa_core.as<float>() + n);
+        __bf16 *hi = c.keys2.as<__bf16>(), *lo = reinterpret_cast<__bf16 *>(c.vals2.p);
+        float *na = c.soa_core.as<float>(), *rho = c.soa_core.as<float>() + n;
+        const bool vec = (d & 3) == 0 && ((uintptr_t)c.coords & 15) == 0;
+        switch (kp) {
+            case 16: k_mfma_prep<16><<<nb, 256, 0, s>>>(c.coords, perm, n, d, vec, mu, c.soa.as<float>(), hi, lo, na, rho); break;
+            case 32: k_mfma_prep<32><<<nb, 256, 0, s>>>(c.coords, perm, n, d, vec, mu, c.soa.as<float>(), hi, lo, na, rho); break;
+            case 64: k_mfma_prep<64><<<nb, 256, 0, s>>>(c.coords, perm, n, d, vec, mu, c.soa.as<float>(), hi, lo, na, rho); break;
+            default: k_mfma_prep<128><<<nb, 256, 0, s>>>(c.coords, perm, n, d, vec, mu, c.soa.as<float>(), hi, lo, na, rho); break;
+        }
     }
     // the padding rows hold zeros (their columns are out of range wherever
     // they are read)
     fill_u64(reinterpret_cast<uint64_t *>(c.keys2.as<__bf16>() + (size_t)n * kp), 0, 8 * kp, s, c.prof);
     fill_u64(reinterpret_cast<uint64_t *>(c.soa_core.as<float>() + 2 * n), 0, 16, s, c.prof);
     c.sync();
-    const int64_t nc = h_total;
     c.n_cells = nc;
     const int ncb = (int)ceil_div(nc, 256);
     // cell radii and largest |a'|
@@ -1535,6 +1844,67 @@ void mfma_classify(Ctx &c) {
         count_groups = c.cgroups.as<int4>();
     }
     c.pair_tests = (int64_t)h_pts;  // point pairs of the candidate cell pairs (read after the syncs above)
+    // block pruning: per tile, its largest radius and the bounds of its
+    // distance to every cell's centroid; per (count group, candidate entry),
+    // the column tiles not pruned.  Left out when the bounds would take more
+    // than kPruneBytes or the mask words overflow int32 offsets.
+    const int64_t nt = c.n_tiles;
+    bool prune = nt > 0 && (size_t)nt * (size_t)nc * 8 <= kPruneBytes;
+    if (prune) {
+        c.mfp_rad.ensure(n * 4);
+        c.mfp_trad.ensure(nt * 4);
+        c.mfp_tab.ensure((size_t)nt * nc * 4);
+        c.mfp_tcm.ensure((size_t)nt * nc * 4);
+        c.mfp_moff.ensure((n_count_groups + 1) * 4);
+        Launch L(c.prof, s, "mf_prune");
+        k_point_radius<<<nb, 256, 0, s>>>(c.soa.as<float>(), c.cell_of.as<int32_t>(), c.mfp_cent.as<float>(), n, d,
+                                          c.mfp_rad.as<float>());
+        switch (kp) {
+            case 16: k_tile_dist<16><<<(unsigned)nt, 256, 0, s>>>(c.soa.as<float>(), c.ctiles.as<int4>(), c.mfp_cent.as<float>(), nc, d, c.mfp_rad.as<float>(), c.mfp_tab.as<float>(), c.mfp_trad.as<float>()); break;
+            case 32: k_tile_dist<32><<<(unsigned)nt, 256, 0, s>>>(c.soa.as<float>(), c.ctiles.as<int4>(), c.mfp_cent.as<float>(), nc, d, c.mfp_rad.as<float>(), c.mfp_tab.as<float>(), c.mfp_trad.as<float>()); break;
+            case 64: k_tile_dist<64><<<(unsigned)nt, 256, 0, s>>>(c.soa.as<float>(), c.ctiles.as<int4>(), c.mfp_cent.as<float>(), nc, d, c.mfp_rad.as<float>(), c.mfp_tab.as<float>(), c.mfp_trad.as<float>()); break;
+            default: k_tile_dist<128><<<(unsigned)nt, 256, 0, s>>>(c.soa.as<float>(), c.ctiles.as<int4>(), c.mfp_cent.as<float>(), nc, d, c.mfp_rad.as<float>(), c.mfp_tab.as<float>(), c.mfp_trad.as<float>()); break;
+        }
+        k_transpose_f32<<<dim3((unsigned)ceil_div(nc, 32), (unsigned)ceil_div(nt, 32)), 256, 0, s>>>(
+            c.mfp_tab.as<float>(), nt, nc, c.mfp_tcm.as<float>());
+        k_group_entries<<<(unsigned)ceil_div(n_count_groups, 256), 256, 0, s>>>(
+            count_groups, n_count_groups, c.tile_off.as<int32_t>(), c.mfp_moff.as<int32_t>());
+    }
+    int32_t h_words = 0;
+    if (prune) {
+        int32_t *d_words = c.misc.as<int32_t>() + 604;
+        // (mask words per group are at most nc, so the scan of int32 counts
+        // cannot wrap before the total is checked: its sum is bounded by
+        // n_count_groups * nc, checked in 64 bits first)
+        if ((int64_t)n_count_groups * nc >= INT32_MAX) prune = false;
+        else {
+            scan_exclusive_i32(c.mfp_moff.as<int32_t>(), c.mfp_moff.as<int32_t>(), n_count_groups, d_words, c.scratch,
+                               s, c.prof);
+            NDBX_HIP(hipMemcpyAsync(&h_words, d_words, 4, hipMemcpyDeviceToHost, s));
+            c.sync();
+        }
+    }
+    if (prune) {
+        c.mfp_mask.ensure((size_t)h_words * 8 + 8);
+        c.mfp_tacc.ensure((size_t)nt * 4);
+        fill_i32(c.mfp_tacc.as<int32_t>(), 0, nt, s, c.prof);
+        unsigned long long *d_kept = reinterpret_cast<unsigned long long *>(c.misc.as<int32_t>() + 608);
+        fill_u64(reinterpret_cast<uint64_t *>(d_kept), 0, 1, s, c.prof);
+        const double rr = std::sqrt((double)e2 * (1.0 + std::ldexp(1.0, -18)));
+        {
+            Launch L(c.prof, s, "mf_prune");
+            k_block_masks<<<(unsigned)n_count_groups, 256, 0, s>>>(
+                count_groups, c.tile_off.as<int32_t>(), c.ranges.as<int32_t>(), c.ctile_off.as<int32_t>(),
+                c.ctiles.as<int4>(), c.mfp_tcm.as<float>(), c.mfp_trad.as<float>(), nt, c.mfp_cent.as<float>(), d,
+                rr, (double)e2, c.mfp_moff.as<int32_t>(), c.mfp_mask.as<uint64_t>(), c.mfp_tacc.as<int32_t>(),
+                c.pflag.as<uint8_t>(), d_kept);
+        }
+        unsigned long long h_kept = 0;
+        NDBX_HIP(hipMemcpyAsync(&h_kept, d_kept, 8, hipMemcpyDeviceToHost, s));
+        c.sync();
+        c.pair_tests = (int64_t)h_kept;  // point pairs of the blocks the count pass computes
+    }
+    c.pruned = prune;
     // count
     c.nn_s.ensure(n * 4);
     MfmaArgs a = mfma_args(c);
@@ -1556,7 +1926,16 @@ void mfma_classify(Ctx &c) {
         MfmaArgs ac = a;
         ac.groups = count_groups;
         ac.n_groups = n_count_groups;
+        if (prune) {
+            ac.bmask = c.mfp_mask.as<uint64_t>();
+            ac.bmask_off = c.mfp_moff.as<int32_t>();
+        }
         launch_count(c, ac, kp);
+    }
+    if (prune) {  // the tiles counted wholly within eps
+        Launch L(c.prof, s, "mf_prune");
+        k_tile_acc<<<nb, 256, 0, s>>>(c.mfp_tacc.as<int32_t>(), c.cell_of.as<int32_t>(), c.cell_start.as<int32_t>(),
+                                      c.ctile_off.as<int32_t>(), n, a.nn_s);
     }
     {
         Launch L(c.prof, s, "mf_band");

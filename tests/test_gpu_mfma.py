@@ -114,3 +114,62 @@ def test_mfma_union_repeatable(gpu_lib):
         np.testing.assert_array_equal(a[0], ref[0])
         np.testing.assert_array_equal(a[1], ref[1])
         assert a[2] == ref[2]
+
+
+def _pruning_mix(seed, d, n_per, eps, far):
+    """Clusters of a few widths (sigma in eps units) at centres `far` apart
+    (in eps units, along random directions), plus uniform background: tight
+    clusters hold tiles wholly within eps, far ones tiles with no pair
+    within eps."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    sigmas = (0.02, 0.05, 0.3, 0.7)
+    for k, sg in enumerate(sigmas * 2):
+        c = rng.normal(0.0, 1.0, size=d)
+        c *= far * eps * (k + 1) / np.linalg.norm(c)
+        parts.append(rng.normal(0.0, sg * eps / np.sqrt(d), size=(n_per, d)) + c)
+    bg = rng.uniform(-far * eps * 9, far * eps * 9, size=(n_per, d))
+    x = np.concatenate(parts + [bg]).astype(np.float32)
+    return x[rng.permutation(len(x))]
+
+
+@pytest.mark.parametrize("d,far", [(16, 1.7), (48, 1.05)])
+def test_mfma_block_pruning_against_oracle(gpu_lib, d, far):
+    """The count pass skips column tiles that the centroid bounds prove hold
+    no pair within eps, and counts without products those that hold only
+    neighbours: every count, label and the meta equal the oracle's, and the
+    pairs computed are far fewer than all pairs."""
+    eps = np.float32(4.0)
+    x = _pruning_mix(77 + d, d, 1500, float(eps), far)
+    n = len(x)
+    print(f"case mfma pruning d={d} far={far} n={n}", flush=True)
+    with gpu_lib.Context(x, eps, 12, flags=gpu_lib.FP64_AUDIT) as c:
+        c.classify().label()
+        nn, lab, meta, audit, st = c.nn(), c.labels(), c.meta(), c.audit(), c.stats()
+    nn_o, lab_o, meta_o = O.dbscan(x, eps, 12)
+    assert audit["filter_flips"] == 0, audit
+    np.testing.assert_array_equal(nn, nn_o)
+    np.testing.assert_array_equal(lab, lab_o)
+    assert meta == meta_o
+    assert 0 < st["pair_tests"] < 0.5 * n * n / 2, st
+
+
+def test_mfma_pruning_near_eps(gpu_lib):
+    """Three clusters whose centres are eps apart: pairs across clusters sit
+    on both sides of eps, so tiles next to the pruning and the wholly-within
+    bounds meet pairs near eps; all counts equal the oracle's."""
+    rng = np.random.default_rng(5)
+    d, eps = 24, np.float32(3.0)
+    cen = np.zeros((3, d))
+    cen[1, 0] = float(eps)
+    cen[2, 1] = float(eps)
+    x = np.concatenate([rng.normal(0.0, 0.15, size=(3000, d)) / np.sqrt(d) * 4 + c for c in cen]).astype(np.float32)
+    x = x[rng.permutation(len(x))]
+    print("case mfma pruning near eps", flush=True)
+    with gpu_lib.Context(x, eps, 10) as c:
+        c.classify().label()
+        nn, lab, meta = c.nn(), c.labels(), c.meta()
+    nn_o, lab_o, meta_o = O.dbscan(x, eps, 10)
+    np.testing.assert_array_equal(nn, nn_o)
+    np.testing.assert_array_equal(lab, lab_o)
+    assert meta == meta_o
