@@ -155,17 +155,25 @@ def traffic_from_profiles(kernel, workload):
 def roofline_for(kernel, avg_s, n, d, stats, sum_nn, workload):
     """Roofline of the dominant kernel from its average launch time."""
     if kernel == "mf_count":
-        # F = 2d flops (a d-term dot product) per unordered point pair of the
-        # candidate cell pairs = d per ordered pair (pair_tests counts ordered
-        # pairs); issued: 3 bf16 MFMAs per 16-wide k step (hi*hi, hi*lo,
-        # lo*hi), k padded to kp, each unordered pair once
+        # F = 2d flops (a d-term dot product) per point pair of the 32 x 32
+        # blocks the count pass computes (pair_tests: rows x columns of those
+        # blocks, each unordered pair of cells once); blocks the centroid
+        # bounds prove empty, or wholly within eps (counted without products),
+        # are not computed.  Issued: up to 3 bf16 MFMAs per 16-wide k step
+        # (hi*hi, hi*lo, lo*hi), k padded to kp.
         kp = 16 if d <= 16 else 32 if d <= 32 else 64 if d <= 64 else 128
-        flops = 1.0 * d * float(stats["pair_tests"])
+        flops = 2.0 * d * float(stats["pair_tests"])
         tf = flops / avg_s / 1e12
+        all_pairs = n * (n - 1) / 2.0
         return {"bound": "mfma", "kernel": kernel, "achieved": tf, "peak": BF16_DENSE_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": tf / BF16_DENSE_PEAK_TFLOPS,
                 "traffic": traffic_from_profiles(kernel, workload), "algorithmic_flops": flops,
-                "basis": "2d flops per unordered pair of candidate cells", "avg_ms": avg_s * 1e3,
+                "basis": "2d flops per point pair of the blocks the count pass computes", "avg_ms": avg_s * 1e3,
+                "computed_pairs": int(stats["pair_tests"]),
+                "computed_fraction_of_all_pairs": float(stats["pair_tests"]) / all_pairs,
+                "all_pairs_tflops": 2.0 * d * all_pairs / avg_s / 1e12,
+                "all_pairs_note": "2d flops per unordered pair of all N(N-1)/2 pairs over the launch: the rate at "
+                                  "which pairs are decided, not arithmetic issued",
                 "issued_mfma_tflops_max": tf * 3.0 * kp / d,
                 "issued_note": "upper bound: every block's three bf16 products issued"}
     ab = algorithmic_bytes(kernel, n, d, stats["cells"])

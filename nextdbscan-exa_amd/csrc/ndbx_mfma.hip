@@ -293,16 +293,18 @@ __global__ __launch_bounds__(256) void k_point_radius(const float *__restrict__ 
 }
 
 // Per row tile t (up to 32 sorted points of one cell) and cell C: tab[t][C],
-// a lower bound of the smallest distance of t's points to C's centroid, and
-// rad_t[t], the largest radius of t's points.  A float32 fused sum of d
+// a lower bound of the smallest distance of t's points to C's centroid;
+// rad_t[t], the largest radius of t's points, and rho_t[t], their largest
+// centred norm |a'| (the guard band's R for a block of t's columns).  A float32 fused sum of d
 // squared rounded differences is within (d + 1) 2^-24 (relative, first
 // order) of the exact square distance, so s (1 - 2^-15) is below it for
 // d <= 128.
 template <int KP>
 __global__ __launch_bounds__(256) void k_tile_dist(const float *__restrict__ xs, const int4 *__restrict__ tiles,
                                                    const float *__restrict__ mc, int64_t nc, int d,
-                                                   const float *__restrict__ rad, float *__restrict__ tab,
-                                                   float *__restrict__ rad_t) {
+                                                   const float *__restrict__ rad, const float *__restrict__ rho,
+                                                   float *__restrict__ tab, float *__restrict__ rad_t,
+                                                   float *__restrict__ rho_t) {
     __shared__ float sp[32 * KP];
     const int64_t t = blockIdx.x;
     const int4 tile = tiles[t];
@@ -313,8 +315,15 @@ __global__ __launch_bounds__(256) void k_tile_dist(const float *__restrict__ xs,
     }
     if (threadIdx.x < 64) {
         float m = threadIdx.x < rows ? rad[tile.y + threadIdx.x] : 0.0f;
-        for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-        if (threadIdx.x == 0) rad_t[t] = m;
+        float h = threadIdx.x < rows ? rho[tile.y + threadIdx.x] : 0.0f;
+        for (int o = 32; o >= 1; o >>= 1) {
+            m = fmaxf(m, __shfl_xor(m, o, 64));
+            h = fmaxf(h, __shfl_xor(h, o, 64));
+        }
+        if (threadIdx.x == 0) {
+            rad_t[t] = m;
+            rho_t[t] = h;
+        }
     }
     __syncthreads();
     for (int64_t C = threadIdx.x; C < nc; C += blockDim.x) {
@@ -598,6 +607,8 @@ struct MfmaArgs {
     const int32_t *cand_off, *cand, *cell_start;
     const uint64_t *bmask;     // count pass: kept column tiles per (group, candidate entry), or null
     const int32_t *bmask_off;  // per group: its first mask word
+    const float *tile_rho;     // count pass: per row tile its largest |a'| (null: per cell, cell_rmax)
+    const int32_t *ctoff;      // cell -> its first row tile
     const __bf16 *hi, *lo;
     const float *na, *rho, *cell_rmax, *xs;
     int64_t n;
@@ -1208,8 +1219,8 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
     bool m1 = ci < c_end;
     fetch(1);
     landed_barrier();
-    int buf = 0, cur_o = -1;
-    float hb = 0.0f, hskip = 0.0f;
+    int buf = 0;
+    float hb = 0.0f, hskip = 0.0f, cur_cr = -1.0f;
     for (;;) {
         // this block's place in the sequence and the next one's
         // (wave-uniform, in scalar registers), then the loads of the block
@@ -1221,9 +1232,13 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
         fetch(buf == 0 ? 2 : buf - 1);
         const bool diag = o == cell;
         if (wave_on) {
-            if (o != cur_o) {
-                cur_o = o;
-                const double R = fmax((double)rmax_row, (double)ld_const(A.cell_rmax + o));
+            // R: the largest centred norm of the rows and of the block's
+            // columns (its tile's, or its cell's without tile bounds)
+            const float cr = A.tile_rho ? ld_const(A.tile_rho + (ld_const(A.ctoff + o) + ((bcb - bcs) >> 5)))
+                                        : ld_const(A.cell_rmax + o);
+            if (cr != cur_cr) {
+                cur_cr = cr;
+                const double R = fmax((double)rmax_row, (double)cr);
                 hb = (float)(A.b1 * R * R + A.b2 * R + A.b3);  // half the band
                 // skip threshold of the hi*hi partial sum: the two products
                 // it leaves out are each at most 2^-8 (1 + 2^-8)^2 |a| |b|
@@ -1853,6 +1868,7 @@ void mfma_classify(Ctx &c) {
     if (prune) {
         c.mfp_rad.ensure(n * 4);
         c.mfp_trad.ensure(nt * 4);
+        c.mfp_trho.ensure(nt * 4);
         c.mfp_tab.ensure((size_t)nt * nc * 4);
         c.mfp_tcm.ensure((size_t)nt * nc * 4);
         c.mfp_moff.ensure((n_count_groups + 1) * 4);
@@ -1860,10 +1876,10 @@ void mfma_classify(Ctx &c) {
         k_point_radius<<<nb, 256, 0, s>>>(c.soa.as<float>(), c.cell_of.as<int32_t>(), c.mfp_cent.as<float>(), n, d,
                                           c.mfp_rad.as<float>());
         switch (kp) {
-            case 16: k_tile_dist<16><<<(unsigned)nt, 256, 0, s>>>(c.soa.as<float>(), c.ctiles.as<int4>(), c.mfp_cent.as<float>(), nc, d, c.mfp_rad.as<float>(), c.mfp_tab.as<float>(), c.mfp_trad.as<float>()); break;
-            case 32: k_tile_dist<32><<<(unsigned)nt, 256, 0, s>>>(c.soa.as<float>(), c.ctiles.as<int4>(), c.mfp_cent.as<float>(), nc, d, c.mfp_rad.as<float>(), c.mfp_tab.as<float>(), c.mfp_trad.as<float>()); break;
-            case 64: k_tile_dist<64><<<(unsigned)nt, 256, 0, s>>>(c.soa.as<float>(), c.ctiles.as<int4>(), c.mfp_cent.as<float>(), nc, d, c.mfp_rad.as<float>(), c.mfp_tab.as<float>(), c.mfp_trad.as<float>()); break;
-            default: k_tile_dist<128><<<(unsigned)nt, 256, 0, s>>>(c.soa.as<float>(), c.ctiles.as<int4>(), c.mfp_cent.as<float>(), nc, d, c.mfp_rad.as<float>(), c.mfp_tab.as<float>(), c.mfp_trad.as<float>()); break;
+            case 16: k_tile_dist<16><<<(unsigned)nt, 256, 0, s>>>(c.soa.as<float>(), c.ctiles.as<int4>(), c.mfp_cent.as<float>(), nc, d, c.mfp_rad.as<float>(), c.soa_core.as<float>() + n, c.mfp_tab.as<float>(), c.mfp_trad.as<float>(), c.mfp_trho.as<float>()); break;
+            case 32: k_tile_dist<32><<<(unsigned)nt, 256, 0, s>>>(c.soa.as<float>(), c.ctiles.as<int4>(), c.mfp_cent.as<float>(), nc, d, c.mfp_rad.as<float>(), c.soa_core.as<float>() + n, c.mfp_tab.as<float>(), c.mfp_trad.as<float>(), c.mfp_trho.as<float>()); break;
+            case 64: k_tile_dist<64><<<(unsigned)nt, 256, 0, s>>>(c.soa.as<float>(), c.ctiles.as<int4>(), c.mfp_cent.as<float>(), nc, d, c.mfp_rad.as<float>(), c.soa_core.as<float>() + n, c.mfp_tab.as<float>(), c.mfp_trad.as<float>(), c.mfp_trho.as<float>()); break;
+            default: k_tile_dist<128><<<(unsigned)nt, 256, 0, s>>>(c.soa.as<float>(), c.ctiles.as<int4>(), c.mfp_cent.as<float>(), nc, d, c.mfp_rad.as<float>(), c.soa_core.as<float>() + n, c.mfp_tab.as<float>(), c.mfp_trad.as<float>(), c.mfp_trho.as<float>()); break;
         }
         k_transpose_f32<<<dim3((unsigned)ceil_div(nc, 32), (unsigned)ceil_div(nt, 32)), 256, 0, s>>>(
             c.mfp_tab.as<float>(), nt, nc, c.mfp_tcm.as<float>());
@@ -1929,6 +1945,8 @@ void mfma_classify(Ctx &c) {
         if (prune) {
             ac.bmask = c.mfp_mask.as<uint64_t>();
             ac.bmask_off = c.mfp_moff.as<int32_t>();
+            ac.tile_rho = c.mfp_trho.as<float>();
+            ac.ctoff = c.ctile_off.as<int32_t>();
         }
         launch_count(c, ac, kp);
     }
