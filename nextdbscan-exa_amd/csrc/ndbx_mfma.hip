@@ -2,8 +2,8 @@
 // and border attachment on 32x32 tiles of bf16 matrix products.
 //
 // 1. Partition: K sampled points are cell centres; every point joins its
-//    nearest centre (float32 distances, lowest index on ties); a radix sort by
-//    cell makes each cell a contiguous range.  A cell's radius R_C (float64,
+//    nearest centre (bf16 matrix-product estimates, lowest index on ties); a
+//    radix sort by cell makes each cell a contiguous range.  A cell's radius R_C (float64,
 //    max member distance to its centre) gives a rigorous skip: cells C, C'
 //    whose centres are farther apart than R_C + R_C' + r can hold no pair
 //    within r = sqrt(e2 (1 + 2^-18)), so their tiles are never computed.
@@ -89,48 +89,88 @@ __global__ __launch_bounds__(256) void k_colsum(const float *__restrict__ x, int
     }
 }
 
-// Nearest centre (float32 distance, lowest index on ties), the point held in
-// registers (zero-padded to KP); centres staged through LDS 64 at a time.
+__device__ __forceinline__ __bf16 to_bf16(float f) { return (__bf16)f; }  // round to nearest even
+
+// The centres' bf16 rows (centred on mu, padded to KP, the padding centres
+// up to a multiple of 32 all zero) and their squared norms (+inf for the
+// padding, which is then never the nearest).
 template <int KP>
-__global__ __launch_bounds__(256) void k_assign(const float *__restrict__ x, int64_t n, int d,
-                                                const float *__restrict__ cent, int k_cent,
-                                                uint64_t *__restrict__ keys, uint32_t *__restrict__ vals) {
-    __shared__ float sc[64 * KP];
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool v = i < n;
-    float p[KP];
-#pragma unroll
-    for (int k = 0; k < KP; ++k) p[k] = (v && k < d) ? x[i * d + k] : 0.0f;
-    float best = __builtin_inff();
-    int arg = 0;
-    for (int c0 = 0; c0 < k_cent; c0 += 64) {
-        const int cn = k_cent - c0 < 64 ? k_cent - c0 : 64;
-        __syncthreads();
-        for (int t = threadIdx.x; t < 64 * KP; t += blockDim.x) {
-            const int cc = t / KP, k = t - cc * KP;
-            sc[t] = (cc < cn && k < d) ? cent[(int64_t)(c0 + cc) * d + k] : 0.0f;
-        }
-        __syncthreads();
-        for (int c = 0; c < cn; ++c) {
-            float sum = 0.0f;
-#pragma unroll
-            for (int k = 0; k < KP; ++k) {
-                const float df = p[k] - sc[c * KP + k];
-                sum = __fadd_rn(sum, __fmul_rn(df, df));
-            }
-            if (sum < best) {
-                best = sum;
-                arg = c0 + c;
-            }
-        }
+__global__ __launch_bounds__(256) void k_centres_bf16(const float *__restrict__ cent, int k_cent, int k_pad, int d,
+                                                      const float *__restrict__ mu, __bf16 *__restrict__ chi,
+                                                      float *__restrict__ cn) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= k_pad) return;
+    float s = 0.0f;
+    for (int k = 0; k < KP; ++k) {
+        const __bf16 h = to_bf16(c < k_cent && k < d ? cent[(int64_t)c * d + k] - mu[k] : 0.0f);
+        chi[(int64_t)c * KP + k] = h;
+        s += (float)h * (float)h;
     }
-    if (v) {
-        keys[i] = (uint64_t)arg;
-        vals[i] = (uint32_t)i;
-    }
+    cn[c] = c < k_cent ? s : __builtin_inff();
 }
 
-__device__ __forceinline__ __bf16 to_bf16(float f) { return (__bf16)f; }  // round to nearest even
+// Nearest centre, approximately: the smallest |c|^2 - 2 a.c over the
+// centres, a and c centred and rounded to bf16, from matrix products (lowest
+// index on ties).  Any assignment is a valid partition (cell radii, pair
+// lists and bounds are computed from the points themselves); the nearest
+// centre only keeps the cells compact.  One wave per 32 points: lane l holds
+// row l % 32 (k elements [16 ks + 8 (l / 32), +8) of each 16-wide k step)
+// and takes column l % 32 of every 32 centres.
+template <int KP>
+__global__ __launch_bounds__(256) void k_assign(const float *__restrict__ x, int64_t n, int d,
+                                                const float *__restrict__ mu, const __bf16 *__restrict__ chi,
+                                                const float *__restrict__ cn, int k_pad,
+                                                uint64_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+    constexpr int KS = KP / 16;
+    const int lane = threadIdx.x & 63, half = lane >> 5, col = lane & 31;
+    const int64_t row0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 32;
+    if (row0 >= n) return;
+    const int64_t r = row0 + col;
+    bf16x8_t a[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int k = 16 * ks + 8 * half + u;
+            a[ks][u] = to_bf16(r < n && k < d ? x[r * d + k] - mu[k] : 0.0f);
+        }
+    const bf16x8_t *c8 = reinterpret_cast<const bf16x8_t *>(chi);
+    float best[16];
+    int arg[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) best[q] = __builtin_inff(), arg[q] = 0;
+    for (int c0 = 0; c0 < k_pad; c0 += 32) {
+        const int cc = c0 + col;
+        f32x16_t acc = {};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ks], c8[(int64_t)cc * (KP / 8) + 2 * ks + half], acc, 0, 0, 0);
+        const float nc = cn[cc];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const float est = nc - 2.0f * acc[q];
+            if (est < best[q]) best[q] = est, arg[q] = cc;
+        }
+    }
+    // accumulator row of register q: 8 (q / 4) + 4 half + q % 4; the smallest
+    // over the half's 32 lanes (columns), lowest index on ties
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        float b = best[q];
+        int ix = arg[q];
+#pragma unroll
+        for (int o = 16; o >= 1; o >>= 1) {
+            const float ob = __shfl_xor(b, o, 64);
+            const int oi = __shfl_xor(ix, o, 64);
+            if (ob < b || (ob == b && oi < ix)) b = ob, ix = oi;
+        }
+        const int64_t row = row0 + 8 * (q >> 2) + 4 * half + (q & 3);
+        if (col == 0 && row < n) {
+            keys[row] = (uint64_t)ix;
+            vals[row] = (uint32_t)row;
+        }
+    }
+}
 
 // Sorted float32 rows, centred bf16 hi/lo rows padded to KP, |a'|^2 in
 // float32 (ascending k) and |a'|.  One thread per row, held in registers:
@@ -1677,12 +1717,29 @@ void mfma_classify(Ctx &c) {
     c.keys2.ensure(std::max<size_t>(n * 8, (size_t)(n + 32) * kp * 2));
     c.vals2.ensure(std::max<size_t>(n * 4, (size_t)n * kp * 2));
     {
+        const int k_pad = (int)ceil_div(k_cent, 32) * 32;
+        c.mfp_chi.ensure((size_t)k_pad * kp * 2 + (size_t)k_pad * 4);
+        __bf16 *chi = c.mfp_chi.as<__bf16>();
+        float *cn = reinterpret_cast<float *>(chi + (size_t)k_pad * kp);
+        const unsigned na = (unsigned)ceil_div(n, 128);  // four waves of 32 points per block
         Launch L(c.prof, s, "mf_assign");
         switch (kp) {
-            case 16: k_assign<16><<<nb, 256, 0, s>>>(c.coords, n, d, cent, (int)k_cent, c.keys.as<uint64_t>(), c.vals.as<uint32_t>()); break;
-            case 32: k_assign<32><<<nb, 256, 0, s>>>(c.coords, n, d, cent, (int)k_cent, c.keys.as<uint64_t>(), c.vals.as<uint32_t>()); break;
-            case 64: k_assign<64><<<nb, 256, 0, s>>>(c.coords, n, d, cent, (int)k_cent, c.keys.as<uint64_t>(), c.vals.as<uint32_t>()); break;
-            default: k_assign<128><<<nb, 256, 0, s>>>(c.coords, n, d, cent, (int)k_cent, c.keys.as<uint64_t>(), c.vals.as<uint32_t>()); break;
+            case 16:
+                k_centres_bf16<16><<<(unsigned)ceil_div(k_pad, 256), 256, 0, s>>>(cent, (int)k_cent, k_pad, d, mu, chi, cn);
+                k_assign<16><<<na, 256, 0, s>>>(c.coords, n, d, mu, chi, cn, k_pad, c.keys.as<uint64_t>(), c.vals.as<uint32_t>());
+                break;
+            case 32:
+                k_centres_bf16<32><<<(unsigned)ceil_div(k_pad, 256), 256, 0, s>>>(cent, (int)k_cent, k_pad, d, mu, chi, cn);
+                k_assign<32><<<na, 256, 0, s>>>(c.coords, n, d, mu, chi, cn, k_pad, c.keys.as<uint64_t>(), c.vals.as<uint32_t>());
+                break;
+            case 64:
+                k_centres_bf16<64><<<(unsigned)ceil_div(k_pad, 256), 256, 0, s>>>(cent, (int)k_cent, k_pad, d, mu, chi, cn);
+                k_assign<64><<<na, 256, 0, s>>>(c.coords, n, d, mu, chi, cn, k_pad, c.keys.as<uint64_t>(), c.vals.as<uint32_t>());
+                break;
+            default:
+                k_centres_bf16<128><<<(unsigned)ceil_div(k_pad, 256), 256, 0, s>>>(cent, (int)k_cent, k_pad, d, mu, chi, cn);
+                k_assign<128><<<na, 256, 0, s>>>(c.coords, n, d, mu, chi, cn, k_pad, c.keys.as<uint64_t>(), c.vals.as<uint32_t>());
+                break;
         }
     }
     int bits = 1;
