@@ -111,6 +111,10 @@ static void run_classify(Ctx &c) {
 
 static void run_label(Ctx &c) {
     if (!c.classified) throw Error(NDBX_ESTATE, "ndbx_label before ndbx_classify");
+    // labels are a function of the last classify, whose working buffers the
+    // label pass consumes (the MFMA path reuses its tile tables as scratch):
+    // a second call keeps the labels it has
+    if (c.labelled) return;
     if (c.n == 0) {
         fill_i32(c.meta.as<int32_t>(), 0, 3 * c.n_inst, c.stream, c.prof);
     } else if (c.chunk_points > 0) {

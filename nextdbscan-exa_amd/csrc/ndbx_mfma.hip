@@ -276,6 +276,32 @@ __global__ __launch_bounds__(256) void k_cell_radius(const float *__restrict__ x
 // column of c, itself included).
 __device__ __forceinline__ bool mf_takes(int c, int o) { return o == c || ((c < o) == (((c + o) & 1) == 0)); }
 
+__device__ __forceinline__ int32_t mf_ld(const int32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ int32_t mf_find(int32_t *parent, int32_t x) {
+    for (;;) {
+        const int32_t p = mf_ld(parent + x);
+        if (p == x) return x;
+        const int32_t gp = mf_ld(parent + p);
+        if (gp != p) atomicCAS(parent + x, p, gp);
+        x = gp;
+    }
+}
+__device__ void mf_unite(int32_t *parent, int32_t a, int32_t b) {
+    for (;;) {
+        a = mf_find(parent, a);
+        b = mf_find(parent, b);
+        if (a == b) return;
+        if (a > b) {
+            const int32_t t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS(parent + b, b, a) == b) return;
+    }
+}
+
 // ------------------------------------------------------ block pruning --
 // Per cell: the centroid of its points (float64 sums, rounded to float32);
 // the float32 centroid is the reference point of every radius and distance
@@ -413,7 +439,9 @@ __global__ __launch_bounds__(256) void k_transpose_f32(const float *__restrict__
 //    k_tile_acc adds to every point of the tile; the pair is flagged for the
 //    label passes;
 //  - otherwise computed by the count pass: bit j of masks[moff[g] + i].
-// Entries A does not take are 0.  Adds the point pairs of the computed
+// Bit j of amasks[moff[g] + i] marks the tiles wholly within eps (the union
+// of their cores needs no products either, k_allin_union).  Entries A does
+// not take are 0 in both.  Adds the point pairs of the computed
 // tiles (rows x columns) to *pairs.  tcm: the bounds cell-major; mc: the
 // centroids.
 __global__ __launch_bounds__(256) void k_block_masks(const int4 *__restrict__ groups, const int32_t *__restrict__ cand_off,
@@ -422,7 +450,8 @@ __global__ __launch_bounds__(256) void k_block_masks(const int4 *__restrict__ gr
                                                      const float *__restrict__ tcm, const float *__restrict__ rad_t,
                                                      int64_t nt, const float *__restrict__ mc, int d, double r,
                                                      double e2_in, const int32_t *__restrict__ moff,
-                                                     uint64_t *__restrict__ masks, int32_t *__restrict__ tacc,
+                                                     uint64_t *__restrict__ masks, uint64_t *__restrict__ amasks,
+                                                     int32_t *__restrict__ tacc,
                                                      uint8_t *__restrict__ pflag,
                                                      unsigned long long *__restrict__ pairs) {
     __shared__ int32_t s_rowadd;
@@ -444,7 +473,7 @@ __global__ __launch_bounds__(256) void k_block_masks(const int4 *__restrict__ gr
     int32_t rowadd = 0;
     for (int32_t i = cb + threadIdx.x; i < ce; i += blockDim.x) {
         const int B = cand[i];
-        uint64_t m = 0;
+        uint64_t m = 0, am = 0;
         if (mf_takes(A, B)) {
             const float *tB = tcm + (int64_t)B * nt;  // tiles to B's centroid
             float tg = __builtin_inff();
@@ -467,6 +496,7 @@ __global__ __launch_bounds__(256) void k_block_masks(const int4 *__restrict__ gr
                 const int cols = tiles[j0 + j].z;
                 if (ub * ub * (1.0 + 1.01 * (d + 3) * 5.9604644775390625e-08) * (1.0 + 1e-12) <= e2_in) {
                     any_in = true;
+                    am |= 1ull << j;
                     rowadd += cols;
                     if (B != A) atomicAdd(tacc + j0 + j, rows);
                 } else {
@@ -478,6 +508,7 @@ __global__ __launch_bounds__(256) void k_block_masks(const int4 *__restrict__ gr
             if (any_in) pflag[i] = 1;
         }
         masks[moff[g] + (i - cb)] = m;
+        amasks[moff[g] + (i - cb)] = am;
     }
     for (int o = 32; o >= 1; o >>= 1) {
         kept += __shfl_xor(kept, o, 64);
@@ -501,6 +532,89 @@ __global__ __launch_bounds__(256) void k_tile_acc(const int32_t *__restrict__ ta
     const int c = cell_of[i];
     const int32_t v = tacc[toff[c] + (int32_t)((i - cell_start[c]) >> 5)];
     if (v) nn_s[i] += v;
+}
+
+// Per row tile: its first core (sorted index), or -1; its flag cleared.
+__global__ __launch_bounds__(256) void k_tile_rep(const int4 *__restrict__ tiles, int64_t nt,
+                                                  const uint8_t *__restrict__ core_s, int32_t *__restrict__ rep,
+                                                  int32_t *__restrict__ tflag) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const int4 tile = tiles[t];
+    int32_t r = -1;
+    for (int32_t i = tile.y; i < tile.y + tile.z; ++i)
+        if (core_s[i]) {
+            r = i;
+            break;
+        }
+    rep[t] = r;
+    tflag[t] = 0;
+}
+
+// The union of the tiles k_block_masks found wholly within eps: per count
+// group g and candidate entry, every such column tile J with a core has all
+// its cores within eps of every core of the group's tiles, so all of them
+// are one set.  The group's first core (r0) is united with J's first core
+// and, once per group, with its other tiles' first cores; the tiles are
+// flagged (k_tile_unite then joins each of their cores to its tile's
+// first).
+__global__ __launch_bounds__(256) void k_allin_union(const int4 *__restrict__ groups, const int32_t *__restrict__ cand_off,
+                                                     const int32_t *__restrict__ cand, const int32_t *__restrict__ toff,
+                                                     const int32_t *__restrict__ moff,
+                                                     const uint64_t *__restrict__ amasks,
+                                                     const int32_t *__restrict__ rep, int32_t *__restrict__ tflag,
+                                                     int32_t *__restrict__ parent) {
+    __shared__ int32_t s_any;
+    const int64_t g = blockIdx.x;
+    const int4 grp = groups[g];
+    int32_t r0 = -1;  // the group's first core
+    for (int u = 0; u < grp.z && r0 < 0; ++u) r0 = rep[grp.y + u];
+    if (r0 < 0) return;  // no core in the group: nothing to unite (the same for every thread)
+    if (threadIdx.x == 0) s_any = 0;
+    __syncthreads();
+    const int32_t cb = cand_off[grp.x], ce = cand_off[grp.x + 1];
+    bool any = false;
+    for (int32_t i = cb + threadIdx.x; i < ce; i += blockDim.x) {
+        uint64_t m = amasks[moff[g] + (i - cb)];
+        if (!m) continue;
+        const int32_t j0 = toff[cand[i]];
+        while (m) {
+            const int32_t J = j0 + __builtin_ctzll(m);
+            m &= m - 1;
+            const int32_t rj = rep[J];
+            if (rj < 0) continue;
+            mf_unite(parent, r0, rj);
+            tflag[J] = 1;
+            any = true;
+        }
+    }
+    if (any) s_any = 1;
+    __syncthreads();
+    if (s_any && threadIdx.x < grp.z) {
+        const int32_t rt = rep[grp.y + threadIdx.x];
+        if (rt >= 0) {
+            mf_unite(parent, r0, rt);
+            tflag[grp.y + threadIdx.x] = 1;
+        }
+    }
+}
+
+// Per sorted point: a core of a flagged tile joins its tile's first core.
+// Only first cores (and their roots) have been united so far, so every
+// other core is still its own root, and the first core has the smaller
+// index: it is hung under it directly (the union-find's roots are the
+// smallest index of their set).
+__global__ __launch_bounds__(256) void k_tile_unite(const uint8_t *__restrict__ core_s,
+                                                    const int32_t *__restrict__ cell_of,
+                                                    const int32_t *__restrict__ cell_start,
+                                                    const int32_t *__restrict__ toff, int64_t n,
+                                                    const int32_t *__restrict__ rep,
+                                                    const int32_t *__restrict__ tflag, int32_t *__restrict__ parent) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !core_s[i]) return;
+    const int c = cell_of[i];
+    const int32_t t = toff[c] + (int32_t)((i - cell_start[c]) >> 5);
+    if (tflag[t] && rep[t] != (int32_t)i) parent[i] = rep[t];
 }
 
 // Per count group: its cell's candidate entries (mask words).
@@ -582,32 +696,6 @@ __global__ __launch_bounds__(256) void k_row_tile_count(const int32_t *__restric
 }
 
 // ------------------------------------------------------------ MFMA pass --
-__device__ __forceinline__ int32_t mf_ld(const int32_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ int32_t mf_find(int32_t *parent, int32_t x) {
-    for (;;) {
-        const int32_t p = mf_ld(parent + x);
-        if (p == x) return x;
-        const int32_t gp = mf_ld(parent + p);
-        if (gp != p) atomicCAS(parent + x, p, gp);
-        x = gp;
-    }
-}
-__device__ void mf_unite(int32_t *parent, int32_t a, int32_t b) {
-    for (;;) {
-        a = mf_find(parent, a);
-        b = mf_find(parent, b);
-        if (a == b) return;
-        if (a > b) {
-            const int32_t t = a;
-            a = b;
-            b = t;
-        }
-        if (atomicCAS(parent + b, b, a) == b) return;
-    }
-}
-
 // The reference's predicate on original coordinates (data_process.cpp:443-447).
 __device__ __forceinline__ bool exact_pred(const float *__restrict__ p, const float *__restrict__ q, int d, float e2) {
     float len = 0.0f;
@@ -820,6 +908,17 @@ __global__ __launch_bounds__(256, (PASS == 1 && KP < 128) ? 3 : 1) void k_mfma_p
         return set_skip && idx < kSameSetCap && ((ss_bits[idx >> 5] >> (idx & 31)) & 1u);
     };
     int buf = 0;  // LDS buffer of the block being computed
+    // PASS 1 with block masks: the column tiles of each pair that the count
+    // pass computed (the others hold no pair within eps, or only such pairs,
+    // whose cores k_allin_union has united); bm: those of the current cell
+    // among its first 64 not yet passed
+    const uint64_t *gm = (PASS == 1 && A.bmask) ? A.bmask + (A.bmask_off[gidx] - c_beg) : nullptr;
+    uint64_t bm = ~0ull;
+    auto to_kept = [&](int &cb_, int cs_) {  // cb_ to the first kept tile at or after it
+        if (!gm || cb_ >= cs_ + 64 * 32) return;
+        bm &= ~((1ull << ((cb_ - cs_) >> 5)) - 1ull);
+        cb_ = bm ? cs_ + 32 * __builtin_ctzll(bm) : cs_ + 64 * 32;
+    };
     // the candidate cells a pass visits (an empty range for the others)
     auto wanted = [&](int ci_, int o) -> bool {
         if (PASS == 0 || PASS == 3) return true;
@@ -841,6 +940,7 @@ __global__ __launch_bounds__(256, (PASS == 1 && KP < 128) ? 3 : 1) void k_mfma_p
         cs_ = A.cell_start[cc_];
         ce_ = A.cell_start[cc_ + 1];
         cb_ = wanted(ci_, cc_) ? cs_ : ce_;
+        if (gm && cb_ < ce_) bm = gm[ci_];  // PASS 1 visits only pairs this cell takes
     };
     // move (ci, cb) to the first block at or after it that has work; ci ==
     // c_end when none is left.  PASS 1 jumps over blocks of the cell's own
@@ -849,6 +949,7 @@ __global__ __launch_bounds__(256, (PASS == 1 && KP < 128) ? 3 : 1) void k_mfma_p
         while (ci_ < c_end) {
             if (PASS == 1 && cb_ + 31 < min_row_point && A.cand[ci_] == cell)
                 cb_ += 32 * ((min_row_point - cb_) / 32);
+            to_kept(cb_, cs_);
             // a pair whose cells have come to share one set since its first block
             if (set_skip && cb_ > cs_ && cb_ < ce_ && ss_pub[buf] == ci_) cb_ = ce_;
             if (cb_ < ce_) return;
@@ -922,7 +1023,11 @@ __global__ __launch_bounds__(256, (PASS == 1 && KP < 128) ? 3 : 1) void k_mfma_p
         buf ^= 1;
         do {  // compute this block ("break" ends it; the barrier below always runs)
             if (!wave_on) break;
-            const double R = fmax((double)rmax_row, (double)A.cell_rmax[cc]);
+            // R: the largest centred norm of the rows and of the block's
+            // columns (its tile's, or its cell's without tile bounds)
+            const float cr = A.tile_rho ? A.tile_rho[A.ctoff[cc] + ((cur_cb - A.cell_start[cc]) >> 5)]
+                                        : A.cell_rmax[cc];
+            const double R = fmax((double)rmax_row, (double)cr);
             const float band = (float)(2.0 * (A.b1 * R * R + A.b2 * R + A.b3));
             const float thr_in = A.e2 - band, thr_out = A.e2 + band;
 
@@ -1959,6 +2064,7 @@ void mfma_classify(Ctx &c) {
     }
     if (prune) {
         c.mfp_mask.ensure((size_t)h_words * 8 + 8);
+        c.mfp_amask.ensure((size_t)h_words * 8 + 8);
         c.mfp_tacc.ensure((size_t)nt * 4);
         fill_i32(c.mfp_tacc.as<int32_t>(), 0, nt, s, c.prof);
         unsigned long long *d_kept = reinterpret_cast<unsigned long long *>(c.misc.as<int32_t>() + 608);
@@ -1969,7 +2075,8 @@ void mfma_classify(Ctx &c) {
             k_block_masks<<<(unsigned)n_count_groups, 256, 0, s>>>(
                 count_groups, c.tile_off.as<int32_t>(), c.ranges.as<int32_t>(), c.ctile_off.as<int32_t>(),
                 c.ctiles.as<int4>(), c.mfp_tcm.as<float>(), c.mfp_trad.as<float>(), nt, c.mfp_cent.as<float>(), d,
-                rr, (double)e2, c.mfp_moff.as<int32_t>(), c.mfp_mask.as<uint64_t>(), c.mfp_tacc.as<int32_t>(),
+                rr, (double)e2, c.mfp_moff.as<int32_t>(), c.mfp_mask.as<uint64_t>(), c.mfp_amask.as<uint64_t>(),
+                c.mfp_tacc.as<int32_t>(),
                 c.pflag.as<uint8_t>(), d_kept);
         }
         unsigned long long h_kept = 0;
@@ -2058,6 +2165,25 @@ void mfma_label(Ctx &c) {
     MfmaArgs a = mfma_args(c);
     a.core_s = c.core_s.as<uint8_t>();
     a.parent = c.parent.as<int32_t>();
+    if (c.pruned) {
+        // the tiles counted wholly within eps: their cores united without
+        // products; the union passes then visit only the count pass's tiles
+        const int64_t nt = c.n_tiles, ng = c.n_label_tiles;
+        c.mfp_trep.ensure((size_t)nt * 8);
+        int32_t *rep = c.mfp_trep.as<int32_t>(), *tflag = rep + nt;
+        Launch L(c.prof, s, "mf_allin_union");
+        k_tile_rep<<<(unsigned)ceil_div(nt, 256), 256, 0, s>>>(c.ctiles.as<int4>(), nt, c.core_s.as<uint8_t>(), rep,
+                                                               tflag);
+        k_allin_union<<<(unsigned)ng, 256, 0, s>>>(c.tiles.as<int4>(), c.tile_off.as<int32_t>(), c.ranges.as<int32_t>(),
+                                                   c.ctile_off.as<int32_t>(), c.mfp_moff.as<int32_t>(),
+                                                   c.mfp_amask.as<uint64_t>(), rep, tflag, c.parent.as<int32_t>());
+        k_tile_unite<<<nb, 256, 0, s>>>(c.core_s.as<uint8_t>(), c.cell_of.as<int32_t>(), c.cell_start.as<int32_t>(),
+                                        c.ctile_off.as<int32_t>(), n, rep, tflag, c.parent.as<int32_t>());
+        a.bmask = c.mfp_mask.as<uint64_t>();
+        a.bmask_off = c.mfp_moff.as<int32_t>();
+        a.tile_rho = c.mfp_trho.as<float>();
+        a.ctoff = c.ctile_off.as<int32_t>();
+    }
     // union: the cells' own pairs, then the other flagged pairs, skipped
     // while both cells' cores already share one set
     a.union_phase = 0;
@@ -2072,6 +2198,11 @@ void mfma_label(Ctx &c) {
     a.union_phase = 1;
     a.cell_rep = c.b_parent.as<int32_t>();
     launch_pass<1>(c, a, kp, "mf_union");
+    // the border pass's groups are its own, and ctile_off is reused below
+    a.bmask = nullptr;
+    a.bmask_off = nullptr;
+    a.tile_rho = nullptr;
+    a.ctoff = nullptr;
     c.comp_min.ensure(n * 4);
     fill_i32(c.comp_min.as<int32_t>(), INT_MAX, n, s, c.prof);
     {

@@ -173,3 +173,22 @@ def test_mfma_pruning_near_eps(gpu_lib):
     np.testing.assert_array_equal(nn, nn_o)
     np.testing.assert_array_equal(lab, lab_o)
     assert meta == meta_o
+
+
+def test_mfma_label_twice(gpu_lib):
+    """A second label call returns the labels of the last classify (the MFMA
+    label pass consumes the classify's tile tables), and a new classify and
+    label give the same result again."""
+    x, eps = synth.clusters64d_1m(12000, seed=1003)
+    print("case mfma label twice", flush=True)
+    with gpu_lib.Context(x, eps, 15) as c:
+        c.classify().label()
+        a = (c.nn(), c.labels(), c.meta())
+        c.label()
+        b = (c.nn(), c.labels(), c.meta())
+        c.classify().label()
+        e = (c.nn(), c.labels(), c.meta())
+    for r in (b, e):
+        np.testing.assert_array_equal(a[0], r[0])
+        np.testing.assert_array_equal(a[1], r[1])
+        assert a[2] == r[2]
