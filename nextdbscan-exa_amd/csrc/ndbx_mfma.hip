@@ -1205,6 +1205,13 @@ constexpr int kCountWaves = 4;
 #define NDBX_MF_COUNT_OCC 4
 #endif
 constexpr int kCountOcc = NDBX_MF_COUNT_OCC;
+// LDS buffers of column blocks per count workgroup: the block in the
+// products and the kCountBufs - 1 after it, in flight.
+#ifndef NDBX_MF_COUNT_BUFS
+#define NDBX_MF_COUNT_BUFS 3
+#endif
+constexpr int kCountBufs = NDBX_MF_COUNT_BUFS;
+static_assert(kCountBufs >= 3 && kCountBufs <= 6, "count pass LDS buffers");
 
 template <int KP, int WAVES>
 __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 2) void k_mf_count(MfmaArgs A, int chunk) {
@@ -1214,10 +1221,10 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
     // Only the hi halves are staged (the lo halves are read from global
     // memory by the few blocks that need the full products), unit-major:
     // unit u of column c at u * 32 + c, so the 32 columns a wave's half
-    // reads for one k step are 32 consecutive slots.  Three buffers: the
-    // block in the products and the next two, loaded straight into LDS.
-    __shared__ bf16x8_t sb_hi[3][UNITS];
-    __shared__ float sb_na[3][32];
+    // reads for one k step are 32 consecutive slots.  kCountBufs buffers:
+    // the block in the products and the next ones, loaded straight into LDS.
+    __shared__ bf16x8_t sb_hi[kCountBufs][UNITS];
+    __shared__ float sb_na[kCountBufs][32];
     __shared__ int32_t sb_cacc[kColWin];  // column counts of the current window
     __shared__ int32_t sb_edge;
     __shared__ int4 sb_q[WAVES][kBandQ];  // per wave: guard-band pairs not yet in the global queue
@@ -1279,11 +1286,12 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
     // the first 64 of a cell those its mask keeps (all without masks)
     const int c_beg = ld_const(A.cand_off + cell), c_end = ld_const(A.cand_off + cell + 1);
     const uint64_t *gmask = A.bmask ? A.bmask + (ld_const(A.bmask_off + gidx) - c_beg) : nullptr;
-    // cursor: cell co (candidate ci), columns [cb, cb + 32) of [cs, ce);
-    // bm: the current cell's kept tiles among its first 64 not yet visited
-    int ci = c_beg - 1, cb = 0, cs = 0, ce = 0, co = -1;
+    // cursor: cell co (candidate ci), columns [cb, cb + 32) of [cs, ce), ct
+    // the cell's first row tile; bm: the current cell's kept tiles among its
+    // first 64 not yet visited
+    int ci = c_beg - 1, cb = 0, cs = 0, ce = 0, co = -1, ct = 0;
     uint64_t bm = 0;
-    auto next_cell = [&]() {  // the next taken cell with blocks to compute (ci = c_end: none)
+    auto next_cell = [&]() {  // the next taken cell with blocks to compute (ci >= c_end: none)
         for (++ci; ci < c_end; ++ci) {
             const int o = ld_const(A.cand + ci);
             if (!mf_takes(cell, o)) continue;
@@ -1295,6 +1303,7 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
             bm = m;
             cb = bm ? s0 + 32 * __builtin_ctzll(bm) : s0 + 64 * 32;
             if (cb >= ce) continue;
+            if (A.tile_rho) ct = ld_const(A.ctoff + o);
             return;
         }
     };
@@ -1307,18 +1316,19 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
         }
         if (cb >= ce) next_cell();
     };
-    // A block's loads go straight to LDS (no registers, so two blocks can be
-    // in flight behind the one in the products): each wave loads KP of its
-    // UNITS hi units, 64 per instruction, and 8 of its 32 norms, so every
-    // wave issues kLoads loads per block and waiting until at most kLoads are
-    // outstanding leaves the previous block landed.  Rows are read 32 at a
+    // A block's loads go straight to LDS (no registers, so kCountBufs - 1
+    // blocks can be in flight behind the one in the products): each wave
+    // loads KP of its UNITS hi units, 64 per instruction, and 8 of its 32
+    // norms, so every wave issues kLoads loads per block, and waiting until
+    // at most (kCountBufs - 2) kLoads are outstanding leaves the oldest block
+    // in flight landed (loads complete in issue order for the counter).  Rows are read 32 at a
     // time from the block's first (from row n past the sequence's end): the
     // hi rows and the norms hold 32 rows of padding.  Columns past their
     // cell's end are given norm +inf (accumulator -inf) where the norm is
     // read.
     constexpr int kOps = (KP + 63) / 64;
     constexpr int kLoads = kOps + 1;
-    static_assert(kLoads < 16, "vmcnt field");
+    static_assert((kCountBufs - 2) * kLoads < 16, "vmcnt field");
     const int64_t n_last = A.n - 1;
     // per-lane byte offsets of the loads within a block
     uint32_t hoff[kOps];
@@ -1348,39 +1358,54 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
     // compiler from moving memory operations across it
     auto landed_barrier = [&]() {
         asm volatile("" ::: "memory");
-        __builtin_amdgcn_s_waitcnt(0x0070 | kLoads);  // vmcnt(kLoads) lgkmcnt(0)
+        __builtin_amdgcn_s_waitcnt(0x0070 | (kCountBufs - 2) * kLoads);  // vmcnt(...) lgkmcnt(0)
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
+    };
+    // The blocks fetched and not yet computed, oldest first (q[0] is in the
+    // products): place in the sequence, R of its columns (the largest
+    // centred norm of its tile, or of its cell without tile bounds), read
+    // when the block is fetched so that the scalar loads are long done when
+    // it is computed; m: the block exists.
+    struct Blk {
+        int o, cb, ci, cs, ce;
+        float cr;
+        bool m;
+    };
+    auto at_cursor = [&]() {
+        Blk b;
+        b.o = co, b.cb = cb, b.ci = ci, b.cs = cs, b.ce = ce;
+        b.m = ci < c_end;
+        b.cr = !b.m ? 0.0f : A.tile_rho ? ld_const(A.tile_rho + (ct + ((cb - cs) >> 5))) : ld_const(A.cell_rmax + co);
+        return b;
     };
     next_cell();
     if (ci >= c_end) return;  // cannot happen: a cell always takes itself
     for (int t = threadIdx.x; t < kColWin; t += blockDim.x) sb_cacc[t] = 0;
     if (threadIdx.x == 0) sb_edge = 0;
+    Blk q[kCountBufs - 1];
+    q[0] = at_cursor();
     fetch(0);
-    // the block in the products (cell o0, columns [cb0, ce0)) and, at the
-    // cursor, the next one (m1: there is one)
-    int o0 = co, cb0 = cb, ci0 = ci, cs0 = cs, ce0 = ce;
-    advance();
-    bool m1 = ci < c_end;
-    fetch(1);
+#pragma unroll
+    for (int j = 1; j < kCountBufs - 1; ++j) {
+        advance();
+        q[j] = at_cursor();
+        fetch(j);
+    }
     landed_barrier();
     int buf = 0;
     float hb = 0.0f, hskip = 0.0f, cur_cr = -1.0f;
     for (;;) {
-        // this block's place in the sequence and the next one's
-        // (wave-uniform, in scalar registers), then the loads of the block
-        // after that, which overlap this block's products
-        const int o = o0, bcb = cb0, bci = ci0, bcs = cs0, bce = ce0;
-        const int o1 = co, cb1 = cb, ci1 = ci, cs1 = cs, ce1 = ce;
+        // this block's place in the sequence (wave-uniform, in scalar
+        // registers), then the loads of the block kCountBufs - 1 places on,
+        // which overlap this block's products and the next ones'
+        const int o = q[0].o, bcb = q[0].cb, bci = q[0].ci, bcs = q[0].cs, bce = q[0].ce;
+        const float cr = q[0].cr;
         advance();
-        const bool m2 = ci < c_end;
-        fetch(buf == 0 ? 2 : buf - 1);
+        const Blk nb = at_cursor();
+        fetch(buf == 0 ? kCountBufs - 1 : buf - 1);
         const bool diag = o == cell;
         if (wave_on) {
-            // R: the largest centred norm of the rows and of the block's
-            // columns (its tile's, or its cell's without tile bounds)
-            const float cr = A.tile_rho ? ld_const(A.tile_rho + (ld_const(A.ctoff + o) + ((bcb - bcs) >> 5)))
-                                        : ld_const(A.cell_rmax + o);
             if (cr != cur_cr) {
                 cur_cr = cr;
                 const double R = fmax((double)rmax_row, (double)cr);
@@ -1507,6 +1532,8 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
         landed_barrier();
         // leaving the window (another cell, the next kColWin columns, or the
         // end): its column counts and pair flag go to global memory
+        const bool m1 = q[1].m;
+        const int o1 = q[1].o, cb1 = q[1].cb, cs1 = q[1].cs;
         if (!m1 || o1 != o || (uint32_t)(cb1 - cs1) / kColWin != (uint32_t)(bcb - bcs) / kColWin) {
             if (!diag) {
                 const int32_t wb = bcs + (int32_t)((uint32_t)(bcb - bcs) / kColWin * kColWin);
@@ -1525,9 +1552,10 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
             __syncthreads();
         }
         if (!m1) break;
-        o0 = o1, cb0 = cb1, ci0 = ci1, cs0 = cs1, ce0 = ce1;
-        m1 = m2;
-        buf = __builtin_amdgcn_readfirstlane(buf == 2 ? 0 : buf + 1);
+#pragma unroll
+        for (int j = 0; j + 1 < kCountBufs - 1; ++j) q[j] = q[j + 1];
+        q[kCountBufs - 2] = nb;
+        buf = __builtin_amdgcn_readfirstlane(buf == kCountBufs - 1 ? 0 : buf + 1);
     }
     // no load into LDS outlives the workgroup
     __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) lgkmcnt(0)
