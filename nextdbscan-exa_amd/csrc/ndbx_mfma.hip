@@ -1194,7 +1194,7 @@ __device__ __forceinline__ int64_t mf_group_of(int64_t b, int chunk) {
 // there would hold up the next block's loads, which wait on the same
 // counter).
 constexpr int kColWin = 2048;
-constexpr int kBandQ = 64;       // guard-band pairs queued per wave in LDS
+constexpr int kBandQ = 32;       // guard-band pairs queued per wave in LDS
 // Row tiles (waves) per count workgroup.  Measured on the 1M 64-D workload:
 // 4 (3 waves per SIMD) 220 ms with 1.0 TB of L2 misses; 8 (2 per SIMD) 284 ms
 // with 0.40 TB; 12 (3 per SIMD) 234 ms with 0.30 TB: the pass is bound by
@@ -1228,6 +1228,7 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
     __shared__ int32_t sb_cacc[kColWin];  // column counts of the current window
     __shared__ int32_t sb_edge;
     __shared__ int4 sb_q[WAVES][kBandQ];  // per wave: guard-band pairs not yet in the global queue
+    __shared__ bf16x8_t sa_lo[WAVES][UNITS];  // per wave: its rows' lo halves, unit-major as the blocks
     const int64_t gidx = mf_group_of(blockIdx.x, chunk);
     if (gidx >= A.n_groups) return;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1241,12 +1242,17 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
     const bf16x8_t *lo8 = reinterpret_cast<const bf16x8_t *>(A.lo);
     const float INF = __builtin_inff();
     // A operand: row `col` of the tile, k elements [16 ks + 8 half, +8);
-    // the lo halves are read again by the blocks that need them
+    // the lo halves wait in LDS (zero past the tile's rows) for the blocks
+    // that take the full products
     const bool arow_ok = col < nrows;
     const int64_t arow = row0 + (arow_ok ? col : 0);
     bf16x8_t ahi[KS];
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) ahi[ks] = arow_ok ? hi8[arow * (KP / 8) + ks * 2 + half] : bf16x8_t{};
+    for (int u = lane; u < UNITS; u += 64) {
+        const int r = u & 31;
+        sa_lo[wave][u] = r < nrows ? lo8[(int64_t)(row0 + r) * (KP / 8) + (u >> 5)] : bf16x8_t{};
+    }
     // accumulator rows: r -> tile row 8 (r/4) + 4 half + (r%4); the start
     // value of every block is -(|a|^2 - e2)/2 per row (-inf for rows past
     // the tile, so dv = -inf: never counted, never in the band)
@@ -1457,7 +1463,7 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
                 const bf16x8_t blo = lo8[bpt * (KP / 8) + ks * 2 + half];
-                const bf16x8_t alo = arow_ok ? lo8[arow * (KP / 8) + ks * 2 + half] : bf16x8_t{};
+                const bf16x8_t alo = lds_read16(&sa_lo[wave][(ks * 2 + half) * 32 + col]);
                 const bf16x8_t bhi = lds_read16(&sb_hi[buf][(ks * 2 + half) * 32 + col]);
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[ks], blo, acc, 0, 0, 0);
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi, acc, 0, 0, 0);
