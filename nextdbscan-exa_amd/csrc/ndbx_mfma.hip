@@ -737,6 +737,7 @@ struct MfmaArgs {
     const int32_t *bmask_off;  // per group: its first mask word
     const float *tile_rho;     // count pass: per row tile its largest |a'| (null: per cell, cell_rmax)
     const int32_t *ctoff;      // cell -> its first row tile
+    const float2 *tile_band;   // count pass: per column tile (hb, hskip) of its R (below)
     const __bf16 *hi, *lo;
     const float *na, *rho, *cell_rmax, *xs;
     int64_t n;
@@ -1175,6 +1176,31 @@ __global__ __launch_bounds__(256, (PASS == 1 && KP < 128) ? 3 : 1) void k_mfma_p
     }
 }
 
+// Half the guard band hb and the hi*hi skip threshold hskip of the count
+// pass for R = the largest centred norm of a column tile (its own, or its
+// cell's without tile bounds), in the arithmetic the count pass used per
+// block.  Both are monotone in R (every operation and rounding is), so the
+// count pass's value for R = max(rows' R, columns' R) is the larger hb and
+// the smaller hskip of the row side's and the column side's, exactly.
+__global__ __launch_bounds__(256) void k_tile_band(const int4 *__restrict__ tiles, int64_t nt,
+                                                   const float *__restrict__ tile_rho,
+                                                   const float *__restrict__ cell_rmax, double b1, double b2,
+                                                   double b3, float2 *__restrict__ band) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const double R = (double)(tile_rho ? tile_rho[t] : cell_rmax[tiles[t].x]);
+    const float hb = (float)(b1 * R * R + b2 * R + b3);
+    band[t] = make_float2(hb, -(float)(3.0 * (double)hb + 0.008 * R * R));
+}
+
+// larger / smaller of two floats of one sign, on their bits (scalar unit)
+__device__ __forceinline__ float max_nonneg(float a, float b) {
+    return __uint_as_float(max(__float_as_uint(a), __float_as_uint(b)));
+}
+__device__ __forceinline__ float min_nonpos(float a, float b) {
+    return __uint_as_float(max(__float_as_uint(a), __float_as_uint(b)));
+}
+
 // ------------------------------------------------------ symmetric count --
 // Groups are taken in chunks of `chunk` consecutive groups (a few cells),
 // chunk q on XCD q % 8 (workgroups go to XCDs round-robin by index), so the
@@ -1267,6 +1293,10 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
     float rmax_row = arow_ok ? A.rho[arow] : 0.0f;
     for (int o = 32; o >= 1; o >>= 1) rmax_row = fmaxf(rmax_row, __shfl_xor(rmax_row, o, 64));
     rmax_row = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(rmax_row)));
+    // the rows' side of hb and hskip (k_tile_band)
+    const double Rr = (double)rmax_row;
+    const float hb_row = (float)(A.b1 * Rr * Rr + A.b2 * Rr + A.b3);
+    const float hs_row = -(float)(3.0 * (double)hb_row + 0.008 * Rr * Rr);
 
     // This wave's queued guard-band pairs to the global queue (one atomic);
     // past its capacity they are decided here.
@@ -1309,7 +1339,7 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
             bm = m;
             cb = bm ? s0 + 32 * __builtin_ctzll(bm) : s0 + 64 * 32;
             if (cb >= ce) continue;
-            if (A.tile_rho) ct = ld_const(A.ctoff + o);
+            ct = ld_const(A.ctoff + o);
             return;
         }
     };
@@ -1369,20 +1399,20 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
         asm volatile("" ::: "memory");
     };
     // The blocks fetched and not yet computed, oldest first (q[0] is in the
-    // products): place in the sequence, R of its columns (the largest
-    // centred norm of its tile, or of its cell without tile bounds), read
-    // when the block is fetched so that the scalar loads are long done when
-    // it is computed; m: the block exists.
+    // products): place in the sequence, the column tile's (hb, hskip), read
+    // when the block is fetched so that the scalar load is long done when it
+    // is computed; m: the block exists.
     struct Blk {
         int o, cb, ci, cs, ce;
-        float cr;
+        float2 band;
         bool m;
     };
     auto at_cursor = [&]() {
         Blk b;
         b.o = co, b.cb = cb, b.ci = ci, b.cs = cs, b.ce = ce;
         b.m = ci < c_end;
-        b.cr = !b.m ? 0.0f : A.tile_rho ? ld_const(A.tile_rho + (ct + ((cb - cs) >> 5))) : ld_const(A.cell_rmax + co);
+        const uint64_t bw = !b.m ? 0ull : ld_const(reinterpret_cast<const uint64_t *>(A.tile_band) + (ct + ((cb - cs) >> 5)));
+        b.band = make_float2(__uint_as_float((uint32_t)bw), __uint_as_float((uint32_t)(bw >> 32)));
         return b;
     };
     next_cell();
@@ -1400,32 +1430,30 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
     }
     landed_barrier();
     int buf = 0;
-    float hb = 0.0f, hskip = 0.0f, cur_cr = -1.0f;
     for (;;) {
         // this block's place in the sequence (wave-uniform, in scalar
         // registers), then the loads of the block kCountBufs - 1 places on,
         // which overlap this block's products and the next ones'
         const int o = q[0].o, bcb = q[0].cb, bci = q[0].ci, bcs = q[0].cs, bce = q[0].ce;
-        const float cr = q[0].cr;
+        const float2 band = q[0].band;
         advance();
         const Blk nb = at_cursor();
         fetch(buf == 0 ? kCountBufs - 1 : buf - 1);
         const bool diag = o == cell;
         if (wave_on) {
-            if (cr != cur_cr) {
-                cur_cr = cr;
-                const double R = fmax((double)rmax_row, (double)cr);
-                hb = (float)(A.b1 * R * R + A.b2 * R + A.b3);  // half the band
-                // skip threshold of the hi*hi partial sum: the two products
-                // it leaves out are each at most 2^-8 (1 + 2^-8)^2 |a| |b|
-                // (|lo| <= 2^-8 |x| componentwise), 0.00787 R^2 together; its
-                // own float32 accumulation error and the full sum's are each
-                // within hb, so a partial below -(3 hb + 0.00787 R^2) leaves
-                // the full dv below -hb.  The 0.008 R^2 taken also covers the
-                // one rounding of the per-column threshold below, at most
-                // 2^-24 (|hskip| + R^2 / 2).
-                hskip = -(float)(3.0 * (double)hb + 0.008 * R * R);
-            }
+            // R = the largest centred norm of the rows and of the block's
+            // columns; hb = (float)(b1 R^2 + b2 R + b3), half the band.
+            // hskip, the skip threshold of the hi*hi partial sum: the two
+            // products it leaves out are each at most 2^-8 (1 + 2^-8)^2 |a|
+            // |b| (|lo| <= 2^-8 |x| componentwise), 0.00787 R^2 together; its
+            // own float32 accumulation error and the full sum's are each
+            // within hb, so a partial below -(3 hb + 0.00787 R^2) leaves the
+            // full dv below -hb.  The 0.008 R^2 taken also covers the one
+            // rounding of the per-column threshold below, at most 2^-24
+            // (|hskip| + R^2 / 2).  Both from the rows' and the column tile's
+            // values (k_tile_band).
+            const float hb = max_nonneg(hb_row, band.x);
+            const float hskip = min_nonpos(hs_row, band.y);
             // dv = (e2 - est)/2 = -(|a|^2 - e2)/2 + a.b - |b|^2/2 (-inf for
             // rows and columns out of range).  The hi*hi partial is taken
             // without the column term -|b|^2/2, which moves to the skip
@@ -2144,8 +2172,16 @@ void mfma_classify(Ctx &c) {
             ac.bmask = c.mfp_mask.as<uint64_t>();
             ac.bmask_off = c.mfp_moff.as<int32_t>();
             ac.tile_rho = c.mfp_trho.as<float>();
-            ac.ctoff = c.ctile_off.as<int32_t>();
         }
+        ac.ctoff = c.ctile_off.as<int32_t>();
+        c.mfp_tband.ensure((size_t)std::max<int64_t>(nt, 1) * 8);
+        {
+            Launch L(c.prof, s, "mf_prune");
+            k_tile_band<<<(unsigned)ceil_div(std::max<int64_t>(nt, 1), 256), 256, 0, s>>>(
+                c.ctiles.as<int4>(), nt, prune ? c.mfp_trho.as<float>() : nullptr, a.cell_rmax, a.b1, a.b2, a.b3,
+                c.mfp_tband.as<float2>());
+        }
+        ac.tile_band = c.mfp_tband.as<float2>();
         launch_count(c, ac, kp);
     }
     if (prune) {  // the tiles counted wholly within eps
