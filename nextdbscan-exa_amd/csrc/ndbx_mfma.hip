@@ -306,16 +306,39 @@ __device__ void mf_unite(int32_t *parent, int32_t a, int32_t b) {
 // Per cell: the centroid of its points (float64 sums, rounded to float32);
 // the float32 centroid is the reference point of every radius and distance
 // bound of the block pruning.
-__global__ __launch_bounds__(128) void k_cell_centroid(const float *__restrict__ x, const uint32_t *__restrict__ perm,
+// One workgroup per cell: P = 256 / d point streams (point b + s + P j to
+// stream s), each summing its points per coordinate, then the streams
+// summed in order (a fixed order: the result is deterministic).
+__global__ __launch_bounds__(256) void k_cell_centroid(const float *__restrict__ x, const uint32_t *__restrict__ perm,
                                                        const int32_t *__restrict__ cell_start, int64_t nc, int d,
                                                        float *__restrict__ mc) {
+    __shared__ double part[256];
     const int64_t c = blockIdx.x;
     if (c >= nc) return;
     const int32_t b = cell_start[c], e = cell_start[c + 1];
-    for (int k = threadIdx.x; k < d; k += blockDim.x) {
-        double s = 0.0;
-        for (int32_t i = b; i < e; ++i) s += (double)x[(int64_t)perm[i] * d + k];
-        mc[c * d + k] = e > b ? (float)(s / (double)(e - b)) : 0.0f;
+    for (int k0 = 0; k0 < d; k0 += 256) {
+        const int dk = d - k0 < 256 ? d - k0 : 256;  // coordinates of this round
+        const int P = 256 / dk;
+        const int s = threadIdx.x / dk, k = k0 + (threadIdx.x - s * dk);
+        double acc = 0.0;
+        if (s < P) {
+            int32_t i = b + s;
+            double a1 = 0.0;
+            for (; i + P < e; i += 2 * P) {
+                acc += (double)x[(int64_t)perm[i] * d + k];
+                a1 += (double)x[(int64_t)perm[i + P] * d + k];
+            }
+            if (i < e) acc += (double)x[(int64_t)perm[i] * d + k];
+            acc += a1;
+        }
+        part[threadIdx.x] = acc;
+        __syncthreads();
+        if (threadIdx.x < dk) {
+            double t = 0.0;
+            for (int q = 0; q < P; ++q) t += part[q * dk + threadIdx.x];
+            mc[c * d + k0 + threadIdx.x] = e > b ? (float)(t / (double)(e - b)) : 0.0f;
+        }
+        __syncthreads();
     }
 }
 
@@ -361,10 +384,12 @@ __global__ __launch_bounds__(256) void k_point_radius(const float *__restrict__ 
 // Per row tile t (up to 32 sorted points of one cell) and cell C: tab[t][C],
 // a lower bound of the smallest distance of t's points to C's centroid;
 // rad_t[t], the largest radius of t's points, and rho_t[t], their largest
-// centred norm |a'| (the guard band's R for a block of t's columns).  A float32 fused sum of d
-// squared rounded differences is within (d + 1) 2^-24 (relative, first
-// order) of the exact square distance, so s (1 - 2^-15) is below it for
-// d <= 128.
+// centred norm |a'| (the guard band's R for a block of t's columns).  Each
+// row's square distance is two fused float32 chains (even and odd k) of d/2
+// squared rounded differences, then added: within (d/2 + 3) 2^-24
+// (relative, first order; every term is nonnegative) of the exact square
+// distance, so s (1 - 2^-15) is below it for d <= 128.  Two rows at a time:
+// four independent chains per thread.
 template <int KP>
 __global__ __launch_bounds__(256) void k_tile_dist(const float *__restrict__ xs, const int4 *__restrict__ tiles,
                                                    const float *__restrict__ mc, int64_t nc, int d,
@@ -397,14 +422,19 @@ __global__ __launch_bounds__(256) void k_tile_dist(const float *__restrict__ xs,
 #pragma unroll
         for (int k = 0; k < KP; ++k) m[k] = k < d ? mc[C * d + k] : 0.0f;
         float best = __builtin_inff();
-        for (int r = 0; r < rows; ++r) {
-            float s = 0.0f;
+        for (int r = 0; r < rows; r += 2) {
+            const float *p0 = sp + r * KP, *p1 = sp + (r + 1 < rows ? r + 1 : r) * KP;
+            float s0a = 0.0f, s0b = 0.0f, s1a = 0.0f, s1b = 0.0f;
 #pragma unroll
-            for (int k = 0; k < KP; ++k) {
-                const float df = sp[r * KP + k] - m[k];
-                s = __fmaf_rn(df, df, s);
+            for (int k = 0; k < KP; k += 2) {
+                const float d0 = p0[k] - m[k], d1 = p0[k + 1] - m[k + 1];
+                const float e0 = p1[k] - m[k], e1 = p1[k + 1] - m[k + 1];
+                s0a = __fmaf_rn(d0, d0, s0a);
+                s0b = __fmaf_rn(d1, d1, s0b);
+                s1a = __fmaf_rn(e0, e0, s1a);
+                s1b = __fmaf_rn(e1, e1, s1b);
             }
-            best = fminf(best, s);
+            best = fminf(best, fminf(s0a + s0b, s1a + s1b));
         }
         tab[t * nc + C] = sqrtf(best * (1.0f - 3.0517578125e-05f)) * (1.0f - 2.0f * 5.9604644775390625e-08f);
     }
@@ -1939,7 +1969,7 @@ void mfma_classify(Ctx &c) {
     c.mfp_cent.ensure((size_t)nc * d * 4);
     {
         Launch L(c.prof, s, "mf_prune");
-        k_cell_centroid<<<(unsigned)nc, 128, 0, s>>>(c.coords, perm, c.cell_start.as<int32_t>(), nc, d,
+        k_cell_centroid<<<(unsigned)nc, 256, 0, s>>>(c.coords, perm, c.cell_start.as<int32_t>(), nc, d,
                                                     c.mfp_cent.as<float>());
         k_radius_keys<<<nb, 256, 0, s>>>(c.coords, perm, c.keys.as<uint64_t>(), c.cell_of.as<int32_t>(),
                                          c.mfp_cent.as<float>(), n, d, c.keys2.as<uint64_t>(), c.vals2.as<uint32_t>());
