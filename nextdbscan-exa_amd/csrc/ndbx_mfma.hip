@@ -176,6 +176,32 @@ __global__ __launch_bounds__(256) void k_assign(const float *__restrict__ x, int
 // float32 (ascending k) and |a'|.  One thread per row, held in registers:
 // the bf16 rows are written 16 bytes at a time, and with vec (d % 4 == 0
 // and x 16-byte aligned) so are the float32 rows.
+// One row's centred coordinates a = v - mu (float32, zero past d) as bf16
+// hi = bf16(a) and lo = bf16(a - hi), its |a|^2 (float32 products and sums in
+// order) and |a| (that norm's root, rounded up by 1e-6).
+template <int KP>
+__device__ __forceinline__ void split_row(const float *v, const float *__restrict__ mu, int d, __bf16 *hi,
+                                          __bf16 *lo, float &na, float &rho) {
+    float s = 0.0f;
+    bf16x8_t h8[KP / 8], l8[KP / 8];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+        const float a = k < d ? v[k] - mu[k] : 0.0f;
+        const __bf16 h = to_bf16(a);
+        h8[k / 8][k % 8] = h;
+        l8[k / 8][k % 8] = to_bf16(a - (float)h);
+        s = __fadd_rn(s, __fmul_rn(a, a));
+    }
+    bf16x8_t *hd = reinterpret_cast<bf16x8_t *>(hi), *ld = reinterpret_cast<bf16x8_t *>(lo);
+#pragma unroll
+    for (int u = 0; u < KP / 8; ++u) {
+        hd[u] = h8[u];
+        ld[u] = l8[u];
+    }
+    na = s;
+    rho = sqrtf(s) * (1.0f + 1e-6f);
+}
+
 template <int KP>
 __global__ __launch_bounds__(256) void k_mfma_prep(const float *__restrict__ x, const uint32_t *__restrict__ perm,
                                                    int64_t n, int d, bool vec, const float *__restrict__ mu,
@@ -208,24 +234,22 @@ __global__ __launch_bounds__(256) void k_mfma_prep(const float *__restrict__ x, 
             }
         }
     }
-    float s = 0.0f;
-    bf16x8_t h8[KP / 8], l8[KP / 8];
+    split_row<KP>(v, mu, d, hi + i * KP, lo + i * KP, na[i], rho[i]);
+}
+
+// Per cell centroid: its centred bf16 hi / lo halves, |m'|^2 and |m'| as the
+// points' (k_mfma_prep), for the matrix-core tile bounds (k_tile_dist).
+template <int KP>
+__global__ __launch_bounds__(256) void k_centroid_prep(const float *__restrict__ mc, int64_t nc, int d,
+                                                       const float *__restrict__ mu, __bf16 *__restrict__ hi,
+                                                       __bf16 *__restrict__ lo, float *__restrict__ na,
+                                                       float *__restrict__ rho) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    float v[KP];
 #pragma unroll
-    for (int k = 0; k < KP; ++k) {
-        const float a = k < d ? v[k] - mu[k] : 0.0f;
-        const __bf16 h = to_bf16(a);
-        h8[k / 8][k % 8] = h;
-        l8[k / 8][k % 8] = to_bf16(a - (float)h);
-        s = __fadd_rn(s, __fmul_rn(a, a));
-    }
-    bf16x8_t *hd = reinterpret_cast<bf16x8_t *>(hi + i * KP), *ld = reinterpret_cast<bf16x8_t *>(lo + i * KP);
-#pragma unroll
-    for (int u = 0; u < KP / 8; ++u) {
-        hd[u] = h8[u];
-        ld[u] = l8[u];
-    }
-    na[i] = s;
-    rho[i] = sqrtf(s) * (1.0f + 1e-6f);
+    for (int k = 0; k < KP; ++k) v[k] = k < d ? mc[c * d + k] : 0.0f;
+    split_row<KP>(v, mu, d, hi + c * KP, lo + c * KP, na[c], rho[c]);
 }
 
 // Per cell: radius (float64, max member distance to the centre point) and
@@ -384,59 +408,82 @@ __global__ __launch_bounds__(256) void k_point_radius(const float *__restrict__ 
 // Per row tile t (up to 32 sorted points of one cell) and cell C: tab[t][C],
 // a lower bound of the smallest distance of t's points to C's centroid;
 // rad_t[t], the largest radius of t's points, and rho_t[t], their largest
-// centred norm |a'| (the guard band's R for a block of t's columns).  Each
-// row's square distance is two fused float32 chains (even and odd k) of d/2
-// squared rounded differences, then added: within (d/2 + 3) 2^-24
-// (relative, first order; every term is nonnegative) of the exact square
-// distance, so s (1 - 2^-15) is below it for d <= 128.  Two rows at a time:
-// four independent chains per thread.
+// centred norm |a'| (the guard band's R for a block of t's columns).  One
+// wave per tile, 32 centroids per step: est = (|a'|^2 + |m'|^2) - 2 a'.m',
+// the product from bf16 hi / lo halves (hi*hi + hi*lo + lo*hi, float32
+// accumulation).  With R >= |a'|, |m'|:
+//  - centring (a' = fl(a - mu), m' likewise) moves the distance by at most
+//    2 u R (u = 2^-24), its square by at most 8.01 u R^2 (distances <= 2R);
+//  - each float32 norm is within (d + 1) u R^2, the product's accumulation
+//    within 3.03 Kp u R^2 and its dropped terms (lo*lo and the lo halves'
+//    own rounding, |lo - (a - hi)| <= 2^-16 |a|) within 3.03 2^-16 R^2, both
+//    doubled in est, and est's two roundings within 6.1 u R^2;
+// so |est - |a - m|^2| <= (2d + 6.06 Kp + 16.3) u R^2 + 6.06 2^-16 R^2, below
+// 1.6e-4 R^2 for d, Kp <= 128, and est - 2^-12 R^2 (1.0001) is a lower bound
+// of the square distance; its root times (1 - 2^-20), of the distance.
 template <int KP>
-__global__ __launch_bounds__(256) void k_tile_dist(const float *__restrict__ xs, const int4 *__restrict__ tiles,
-                                                   const float *__restrict__ mc, int64_t nc, int d,
-                                                   const float *__restrict__ rad, const float *__restrict__ rho,
+__global__ __launch_bounds__(256) void k_tile_dist(const __bf16 *__restrict__ hi, const __bf16 *__restrict__ lo,
+                                                   const float *__restrict__ na, const float *__restrict__ rho,
+                                                   const int4 *__restrict__ tiles, int64_t nt,
+                                                   const __bf16 *__restrict__ chi, const __bf16 *__restrict__ clo,
+                                                   const float *__restrict__ cna, const float *__restrict__ crho,
+                                                   int64_t nc, const float *__restrict__ rad,
                                                    float *__restrict__ tab, float *__restrict__ rad_t,
                                                    float *__restrict__ rho_t) {
-    __shared__ float sp[32 * KP];
-    const int64_t t = blockIdx.x;
+    constexpr int KS = KP / 16;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t t = (int64_t)blockIdx.x * 4 + wave;
+    if (t >= nt) return;
+    const int half = lane >> 5, col = lane & 31;
     const int4 tile = tiles[t];
     const int rows = tile.z;
-    for (int u = threadIdx.x; u < 32 * KP; u += blockDim.x) {
-        const int r = u / KP, k = u - r * KP;
-        sp[u] = (r < rows && k < d) ? xs[(int64_t)(tile.y + r) * d + k] : 0.0f;
-    }
-    if (threadIdx.x < 64) {
-        float m = threadIdx.x < rows ? rad[tile.y + threadIdx.x] : 0.0f;
-        float h = threadIdx.x < rows ? rho[tile.y + threadIdx.x] : 0.0f;
-        for (int o = 32; o >= 1; o >>= 1) {
-            m = fmaxf(m, __shfl_xor(m, o, 64));
-            h = fmaxf(h, __shfl_xor(h, o, 64));
-        }
-        if (threadIdx.x == 0) {
-            rad_t[t] = m;
-            rho_t[t] = h;
-        }
-    }
-    __syncthreads();
-    for (int64_t C = threadIdx.x; C < nc; C += blockDim.x) {
-        float m[KP];
+    const bf16x8_t *hi8 = reinterpret_cast<const bf16x8_t *>(hi), *lo8 = reinterpret_cast<const bf16x8_t *>(lo);
+    const bf16x8_t *chi8 = reinterpret_cast<const bf16x8_t *>(chi), *clo8 = reinterpret_cast<const bf16x8_t *>(clo);
+    const bool arow_ok = col < rows;
+    const int64_t arow = tile.y + (arow_ok ? col : 0);
+    bf16x8_t ahi[KS], alo[KS];
 #pragma unroll
-        for (int k = 0; k < KP; ++k) m[k] = k < d ? mc[C * d + k] : 0.0f;
-        float best = __builtin_inff();
-        for (int r = 0; r < rows; r += 2) {
-            const float *p0 = sp + r * KP, *p1 = sp + (r + 1 < rows ? r + 1 : r) * KP;
-            float s0a = 0.0f, s0b = 0.0f, s1a = 0.0f, s1b = 0.0f;
+    for (int ks = 0; ks < KS; ++ks) {
+        ahi[ks] = arow_ok ? hi8[arow * (KP / 8) + ks * 2 + half] : bf16x8_t{};
+        alo[ks] = arow_ok ? lo8[arow * (KP / 8) + ks * 2 + half] : bf16x8_t{};
+    }
+    float m = arow_ok && half == 0 ? rad[arow] : 0.0f;
+    float h = arow_ok ? rho[arow] : 0.0f;
+    for (int o = 32; o >= 1; o >>= 1) {
+        m = fmaxf(m, __shfl_xor(m, o, 64));
+        h = fmaxf(h, __shfl_xor(h, o, 64));
+    }
+    if (lane == 0) {
+        rad_t[t] = m;
+        rho_t[t] = h;
+    }
+    // rows held in the accumulator: r -> tile row 8 (r/4) + 4 half + (r%4)
+    float na_r[16];
 #pragma unroll
-            for (int k = 0; k < KP; k += 2) {
-                const float d0 = p0[k] - m[k], d1 = p0[k + 1] - m[k + 1];
-                const float e0 = p1[k] - m[k], e1 = p1[k + 1] - m[k + 1];
-                s0a = __fmaf_rn(d0, d0, s0a);
-                s0b = __fmaf_rn(d1, d1, s0b);
-                s1a = __fmaf_rn(e0, e0, s1a);
-                s1b = __fmaf_rn(e1, e1, s1b);
-            }
-            best = fminf(best, fminf(s0a + s0b, s1a + s1b));
+    for (int r = 0; r < 16; ++r) {
+        const int tr = 8 * (r >> 2) + 4 * half + (r & 3);
+        na_r[r] = tr < rows ? na[tile.y + tr] : __builtin_inff();
+    }
+    for (int64_t cb = 0; cb < nc; cb += 32) {
+        const int64_t C = cb + col < nc ? cb + col : nc - 1;
+        f32x16_t acc = {};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const bf16x8_t bh = chi8[C * (KP / 8) + ks * 2 + half];
+            const bf16x8_t bl = clo8[C * (KP / 8) + ks * 2 + half];
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[ks], bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[ks], bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[ks], bh, acc, 0, 0, 0);
         }
-        tab[t * nc + C] = sqrtf(best * (1.0f - 3.0517578125e-05f)) * (1.0f - 2.0f * 5.9604644775390625e-08f);
+        const float nb = cna[C];
+        float e = __builtin_inff();
+#pragma unroll
+        for (int r = 0; r < 16; ++r) e = fminf(e, __fsub_rn(__fadd_rn(na_r[r], nb), 2.0f * acc[r]));
+        e = fminf(e, __shfl_xor(e, 32, 64));
+        const float R = fmaxf(h, crho[C]);
+        const float lb = e - 2.44140625e-4f * (R * R) * 1.0001f;
+        if (half == 0 && cb + col < nc)
+            tab[t * nc + cb + col] = lb > 0.0f ? sqrtf(lb) * (1.0f - 9.5367431640625e-07f) : 0.0f;
     }
 }
 
@@ -2129,12 +2176,27 @@ void mfma_classify(Ctx &c) {
         Launch L(c.prof, s, "mf_prune");
         k_point_radius<<<nb, 256, 0, s>>>(c.soa.as<float>(), c.cell_of.as<int32_t>(), c.mfp_cent.as<float>(), n, d,
                                           c.mfp_rad.as<float>());
+        // the centroids centred and split as the points, then the bounds
+        c.mfp_mhi.ensure((size_t)nc * kp * 2);
+        c.mfp_mlo.ensure((size_t)nc * kp * 2);
+        c.mfp_mna.ensure((size_t)nc * 8);
+        __bf16 *mhi = c.mfp_mhi.as<__bf16>(), *mlo = c.mfp_mlo.as<__bf16>();
+        float *cna = c.mfp_mna.as<float>(), *crho = cna + nc;
+        const __bf16 *phi = c.keys2.as<__bf16>(), *plo = reinterpret_cast<const __bf16 *>(c.vals2.p);
+        const float *pna = c.soa_core.as<float>(), *prho = c.soa_core.as<float>() + n;
+        const unsigned gc = (unsigned)ceil_div(nc, 256), gt = (unsigned)ceil_div(nt, 4);
+#define NDBX_TILE_DIST(KPV)                                                                                    \
+    k_centroid_prep<KPV><<<gc, 256, 0, s>>>(c.mfp_cent.as<float>(), nc, d, mu, mhi, mlo, cna, crho);           \
+    k_tile_dist<KPV><<<gt, 256, 0, s>>>(phi, plo, pna, prho, c.ctiles.as<int4>(), nt, mhi, mlo, cna, crho, nc, \
+                                        c.mfp_rad.as<float>(), c.mfp_tab.as<float>(), c.mfp_trad.as<float>(),  \
+                                        c.mfp_trho.as<float>())
         switch (kp) {
-            case 16: k_tile_dist<16><<<(unsigned)nt, 256, 0, s>>>(c.soa.as<float>(), c.ctiles.as<int4>(), c.mfp_cent.as<float>(), nc, d, c.mfp_rad.as<float>(), c.soa_core.as<float>() + n, c.mfp_tab.as<float>(), c.mfp_trad.as<float>(), c.mfp_trho.as<float>()); break;
-            case 32: k_tile_dist<32><<<(unsigned)nt, 256, 0, s>>>(c.soa.as<float>(), c.ctiles.as<int4>(), c.mfp_cent.as<float>(), nc, d, c.mfp_rad.as<float>(), c.soa_core.as<float>() + n, c.mfp_tab.as<float>(), c.mfp_trad.as<float>(), c.mfp_trho.as<float>()); break;
-            case 64: k_tile_dist<64><<<(unsigned)nt, 256, 0, s>>>(c.soa.as<float>(), c.ctiles.as<int4>(), c.mfp_cent.as<float>(), nc, d, c.mfp_rad.as<float>(), c.soa_core.as<float>() + n, c.mfp_tab.as<float>(), c.mfp_trad.as<float>(), c.mfp_trho.as<float>()); break;
-            default: k_tile_dist<128><<<(unsigned)nt, 256, 0, s>>>(c.soa.as<float>(), c.ctiles.as<int4>(), c.mfp_cent.as<float>(), nc, d, c.mfp_rad.as<float>(), c.soa_core.as<float>() + n, c.mfp_tab.as<float>(), c.mfp_trad.as<float>(), c.mfp_trho.as<float>()); break;
+            case 16: NDBX_TILE_DIST(16); break;
+            case 32: NDBX_TILE_DIST(32); break;
+            case 64: NDBX_TILE_DIST(64); break;
+            default: NDBX_TILE_DIST(128); break;
         }
+#undef NDBX_TILE_DIST
         k_transpose_f32<<<dim3((unsigned)ceil_div(nc, 32), (unsigned)ceil_div(nt, 32)), 256, 0, s>>>(
             c.mfp_tab.as<float>(), nt, nc, c.mfp_tcm.as<float>());
         k_group_entries<<<(unsigned)ceil_div(n_count_groups, 256), 256, 0, s>>>(
