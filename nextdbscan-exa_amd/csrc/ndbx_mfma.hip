@@ -812,6 +812,8 @@ struct MfmaArgs {
     const int32_t *cand_off, *cand, *cell_start;
     const uint64_t *bmask;     // count pass: kept column tiles per (group, candidate entry), or null
     const int32_t *bmask_off;  // per group: its first mask word
+    uint64_t *emask;           // count pass: per mask word, the computed tiles holding a pair within eps or a
+                               // guard-band element (zero on entry; the union passes walk these)
     const float *tile_rho;     // count pass: per row tile its largest |a'| (null: per cell, cell_rmax)
     const int32_t *ctoff;      // cell -> its first row tile
     const float2 *tile_band;   // count pass: per column tile (hb, hskip) of its R (below)
@@ -987,9 +989,10 @@ __global__ __launch_bounds__(256, (PASS == 1 && KP < 128) ? 3 : 1) void k_mfma_p
     };
     int buf = 0;  // LDS buffer of the block being computed
     // PASS 1 with block masks: the column tiles of each pair that the count
-    // pass computed (the others hold no pair within eps, or only such pairs,
-    // whose cores k_allin_union has united); bm: those of the current cell
-    // among its first 64 not yet passed
+    // pass computed and found to hold a pair within eps or a guard-band
+    // element (the others hold no pair within eps, or only such pairs, whose
+    // cores k_allin_union has united); bm: those of the current cell among
+    // its first 64 not yet passed
     const uint64_t *gm = (PASS == 1 && A.bmask) ? A.bmask + (A.bmask_off[gidx] - c_beg) : nullptr;
     uint64_t bm = ~0ull;
     auto to_kept = [&](int &cb_, int cs_) {  // cb_ to the first kept tile at or after it
@@ -1020,6 +1023,33 @@ __global__ __launch_bounds__(256, (PASS == 1 && KP < 128) ? 3 : 1) void k_mfma_p
         cb_ = wanted(ci_, cc_) ? cs_ : ce_;
         if (gm && cb_ < ce_) bm = gm[ci_];  // PASS 1 visits only pairs this cell takes
     };
+    // The candidate entries worth loading (wanted, and with a tile to visit
+    // under the masks), 64 at a time, one per lane: win holds those of the
+    // window [win_base, win_base + 64).  Every input is fixed while the pass
+    // runs (pair flags, masks, cell sizes, the same-set bits taken at start),
+    // so every wave finds the same entries.
+    int win_base = -1;
+    uint64_t win = 0;
+    auto next_worth = [&](int from) -> int {  // the first such entry >= from, or c_end
+        while (from < c_end) {
+            const int wb = c_beg + ((from - c_beg) & ~63);
+            if (wb != win_base) {
+                const int e = wb + lane;
+                bool w = false;
+                if (e < c_end) {
+                    const int o = A.cand[e];
+                    w = wanted(e, o);
+                    if (w && gm) w = gm[e] != 0 || A.cell_start[o + 1] - A.cell_start[o] > 64 * 32;
+                }
+                win = __ballot(w);
+                win_base = wb;
+            }
+            const uint64_t m = win & (~0ull << (from - wb));
+            if (m) return wb + __builtin_ctzll(m);
+            from = wb + 64;
+        }
+        return c_end;
+    };
     // move (ci, cb) to the first block at or after it that has work; ci ==
     // c_end when none is left.  PASS 1 jumps over blocks of the cell's own
     // pair that lie entirely below every row of the group.
@@ -1031,12 +1061,13 @@ __global__ __launch_bounds__(256, (PASS == 1 && KP < 128) ? 3 : 1) void k_mfma_p
             // a pair whose cells have come to share one set since its first block
             if (set_skip && cb_ > cs_ && cb_ < ce_ && ss_pub[buf] == ci_) cb_ = ce_;
             if (cb_ < ce_) return;
-            ++ci_;
+            ci_ = next_worth(ci_ + 1);
             if (ci_ < c_end) load_cell(ci_, cb_, cs_, ce_);
         }
     };
     auto advance = [&](int &ci_, int &cb_, int &cs_, int &ce_, bool first) {
         if (first) {
+            ci_ = next_worth(ci_);
             if (ci_ < c_end) load_cell(ci_, cb_, cs_, ce_);
         } else {
             cb_ += 32;
@@ -1330,6 +1361,7 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
     __shared__ float sb_na[kCountBufs][32];
     __shared__ int32_t sb_cacc[kColWin];  // column counts of the current window
     __shared__ int32_t sb_edge;
+    __shared__ unsigned long long sb_emask;  // edge tiles of the current candidate among its first 64
     __shared__ int4 sb_q[WAVES][kBandQ];  // per wave: guard-band pairs not yet in the global queue
     __shared__ bf16x8_t sa_lo[WAVES][UNITS];  // per wave: its rows' lo halves, unit-major as the blocks
     const int64_t gidx = mf_group_of(blockIdx.x, chunk);
@@ -1399,6 +1431,7 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
     // the first 64 of a cell those its mask keeps (all without masks)
     const int c_beg = ld_const(A.cand_off + cell), c_end = ld_const(A.cand_off + cell + 1);
     const uint64_t *gmask = A.bmask ? A.bmask + (ld_const(A.bmask_off + gidx) - c_beg) : nullptr;
+    uint64_t *emask_g = A.bmask && A.emask ? A.emask + (ld_const(A.bmask_off + gidx) - c_beg) : nullptr;
     // cursor: cell co (candidate ci), columns [cb, cb + 32) of [cs, ce), ct
     // the cell's first row tile; bm: the current cell's kept tiles among its
     // first 64 not yet visited
@@ -1495,7 +1528,10 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
     next_cell();
     if (ci >= c_end) return;  // cannot happen: a cell always takes itself
     for (int t = threadIdx.x; t < kColWin; t += blockDim.x) sb_cacc[t] = 0;
-    if (threadIdx.x == 0) sb_edge = 0;
+    if (threadIdx.x == 0) {
+        sb_edge = 0;
+        sb_emask = 0;
+    }
     Blk q[kCountBufs - 1];
     q[0] = at_cursor();
     fetch(0);
@@ -1631,8 +1667,12 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
                     qn += total;
                 }
             }
-            // the pair holds a pair within eps: the label passes visit it
-            if (__ballot(ccnt != 0) != 0 && lane == 0) sb_edge = 1;
+            // the pair holds a pair within eps: the label passes visit it;
+            // the tile holds one, or a guard-band element: the union does
+            const bool edge = __ballot(ccnt != 0) != 0;
+            if (edge && lane == 0) sb_edge = 1;
+            if ((edge || __ballot(amin <= hb) != 0) && lane == 0 && bcb - bcs < 64 * 32)
+                atomicOr(&sb_emask, 1ull << ((bcb - bcs) >> 5));
             if (!diag) {
                 ccnt += __shfl_xor(ccnt, 32, 64);
                 if (half == 0 && ccnt != 0) atomicAdd(&sb_cacc[(bcb - bcs) % kColWin + col], ccnt);
@@ -1659,6 +1699,10 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
             if (threadIdx.x == 0 && sb_edge) {
                 A.pflag[bci] = 1;
                 sb_edge = 0;
+            }
+            if (threadIdx.x == 0 && sb_emask) {
+                if (emask_g) atomicOr(reinterpret_cast<unsigned long long *>(emask_g + bci), sb_emask);
+                sb_emask = 0;
             }
             __syncthreads();
         }
@@ -2219,6 +2263,8 @@ void mfma_classify(Ctx &c) {
     if (prune) {
         c.mfp_mask.ensure((size_t)h_words * 8 + 8);
         c.mfp_amask.ensure((size_t)h_words * 8 + 8);
+        c.mfp_emask.ensure((size_t)h_words * 8 + 8);
+        fill_u64(c.mfp_emask.as<uint64_t>(), 0, h_words, s, c.prof);
         c.mfp_tacc.ensure((size_t)nt * 4);
         fill_i32(c.mfp_tacc.as<int32_t>(), 0, nt, s, c.prof);
         unsigned long long *d_kept = reinterpret_cast<unsigned long long *>(c.misc.as<int32_t>() + 608);
@@ -2263,6 +2309,7 @@ void mfma_classify(Ctx &c) {
         if (prune) {
             ac.bmask = c.mfp_mask.as<uint64_t>();
             ac.bmask_off = c.mfp_moff.as<int32_t>();
+            ac.emask = c.mfp_emask.as<uint64_t>();
             ac.tile_rho = c.mfp_trho.as<float>();
         }
         ac.ctoff = c.ctile_off.as<int32_t>();
@@ -2341,7 +2388,7 @@ void mfma_label(Ctx &c) {
                                                    c.mfp_amask.as<uint64_t>(), rep, tflag, c.parent.as<int32_t>());
         k_tile_unite<<<nb, 256, 0, s>>>(c.core_s.as<uint8_t>(), c.cell_of.as<int32_t>(), c.cell_start.as<int32_t>(),
                                         c.ctile_off.as<int32_t>(), n, rep, tflag, c.parent.as<int32_t>());
-        a.bmask = c.mfp_mask.as<uint64_t>();
+        a.bmask = c.mfp_emask.as<uint64_t>();  // the tiles holding a pair within eps (or a band element)
         a.bmask_off = c.mfp_moff.as<int32_t>();
         a.tile_rho = c.mfp_trho.as<float>();
         a.ctoff = c.ctile_off.as<int32_t>();
