@@ -1437,8 +1437,33 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && KP < 128) ? kCountOcc : 
     // first 64 not yet visited
     int ci = c_beg - 1, cb = 0, cs = 0, ce = 0, co = -1, ct = 0;
     uint64_t bm = 0;
+    // The candidate entries with blocks to compute, 64 at a time, one per
+    // lane (win: those of [win_base, win_base + 64)); the entry found is
+    // then read again with scalar loads.
+    int win_base = -1;
+    uint64_t win = 0;
     auto next_cell = [&]() {  // the next taken cell with blocks to compute (ci >= c_end: none)
         for (++ci; ci < c_end; ++ci) {
+            const int wb = c_beg + ((ci - c_beg) & ~63);
+            if (wb != win_base) {
+                const int e = wb + lane;
+                bool w = false;
+                if (e < c_end) {
+                    const int o = A.cand[e];
+                    if (mf_takes(cell, o)) {
+                        const int s0 = A.cell_start[o], s1 = A.cell_start[o + 1];
+                        w = s0 < s1 && (s1 - s0 > 64 * 32 || !gmask || gmask[e] != 0);
+                    }
+                }
+                win = __ballot(w);
+                win_base = wb;
+            }
+            const uint64_t wm = win & (~0ull << (ci - wb));
+            if (wm == 0) {
+                ci = wb + 63;
+                continue;
+            }
+            ci = wb + __builtin_ctzll(wm);
             const int o = ld_const(A.cand + ci);
             if (!mf_takes(cell, o)) continue;
             const int s0 = ld_const(A.cell_start + o), s1 = ld_const(A.cell_start + o + 1);
