@@ -519,13 +519,13 @@ __global__ __launch_bounds__(256) void k_transpose_f32(const float *__restrict__
 // Bit j of amasks[moff[g] + i] marks the tiles wholly within eps (the union
 // of their cores needs no products either, k_allin_union).  Entries A does
 // not take are 0 in both.  Adds the point pairs of the computed
-// tiles (rows x columns) to *pairs.  tcm: the bounds cell-major; mc: the
-// centroids.
+// tiles (rows x columns) to *pairs.  tcm: the bounds cell-major; dab_e: per
+// candidate entry the upper bound of |m_A - m_B| (k_pair_cdist).
 __global__ __launch_bounds__(256) void k_block_masks(const int4 *__restrict__ groups, const int32_t *__restrict__ cand_off,
                                                      const int32_t *__restrict__ cand,
                                                      const int32_t *__restrict__ toff, const int4 *__restrict__ tiles,
                                                      const float *__restrict__ tcm, const float *__restrict__ rad_t,
-                                                     int64_t nt, const float *__restrict__ mc, int d, double r,
+                                                     int64_t nt, const double *__restrict__ dab_e, int d, double r,
                                                      double e2_in, const int32_t *__restrict__ moff,
                                                      uint64_t *__restrict__ masks, uint64_t *__restrict__ amasks,
                                                      int32_t *__restrict__ tacc,
@@ -545,7 +545,6 @@ __global__ __launch_bounds__(256) void k_block_masks(const int4 *__restrict__ gr
     __syncthreads();
     const int32_t cb = cand_off[A], ce = cand_off[A + 1];
     const float *tA = tcm + (int64_t)A * nt;  // tiles to A's centroid
-    const float *mA = mc + (int64_t)A * d;
     unsigned long long kept = 0;
     int32_t rowadd = 0;
     for (int32_t i = cb + threadIdx.x; i < ce; i += blockDim.x) {
@@ -555,13 +554,7 @@ __global__ __launch_bounds__(256) void k_block_masks(const int4 *__restrict__ gr
             const float *tB = tcm + (int64_t)B * nt;  // tiles to B's centroid
             float tg = __builtin_inff();
             for (int u = 0; u < grp.z; ++u) tg = fminf(tg, tB[grp.y + u]);
-            double dd = 0.0;
-            const float *mB = mc + (int64_t)B * d;
-            for (int k = 0; k < d; ++k) {
-                const double df = (double)mA[k] - (double)mB[k];
-                dd += df * df;
-            }
-            const double dab = sqrt(dd) * (1.0 + 1e-12);  // above |m_A - m_B|
+            const double dab = dab_e[i];  // above |m_A - m_B| (k_pair_cdist)
             const int32_t j0 = toff[B], j1 = toff[B + 1];
             const int nj = j1 - j0 < 64 ? j1 - j0 : 64;
             bool any_in = false;
@@ -597,6 +590,28 @@ __global__ __launch_bounds__(256) void k_block_masks(const int4 *__restrict__ gr
     }
     __syncthreads();
     if (threadIdx.x < grp.z && s_rowadd) atomicAdd(tacc + grp.y + threadIdx.x, s_rowadd);
+}
+
+// Per candidate entry i of cell A (candidate cell B = cand[i]): an upper
+// bound of the centroids' distance |m_A - m_B| (float64 sum of squares in
+// ascending k, root times 1 + 1e-12), for k_block_masks.  One workgroup per
+// cell, threads over its entries.
+__global__ __launch_bounds__(256) void k_pair_cdist(const int32_t *__restrict__ cand_off,
+                                                    const int32_t *__restrict__ cand, int64_t nc,
+                                                    const float *__restrict__ mc, int d, double *__restrict__ dab) {
+    const int64_t A = blockIdx.x;
+    if (A >= nc) return;
+    const int32_t cb = cand_off[A], ce = cand_off[A + 1];
+    const float *mA = mc + A * d;
+    for (int32_t i = cb + threadIdx.x; i < ce; i += blockDim.x) {
+        const float *mB = mc + (int64_t)cand[i] * d;
+        double dd = 0.0;
+        for (int k = 0; k < d; ++k) {
+            const double df = (double)mA[k] - (double)mB[k];
+            dd += df * df;
+        }
+        dab[i] = sqrt(dd) * (1.0 + 1e-12);
+    }
 }
 
 // Per sorted point: its row tile's sum from k_block_masks added to its count.
@@ -2295,11 +2310,14 @@ void mfma_classify(Ctx &c) {
         unsigned long long *d_kept = reinterpret_cast<unsigned long long *>(c.misc.as<int32_t>() + 608);
         fill_u64(reinterpret_cast<uint64_t *>(d_kept), 0, 1, s, c.prof);
         const double rr = std::sqrt((double)e2 * (1.0 + std::ldexp(1.0, -18)));
+        c.mfp_dab.ensure((size_t)h_pairs * 8 + 8);
         {
             Launch L(c.prof, s, "mf_prune");
+            k_pair_cdist<<<(unsigned)nc, 256, 0, s>>>(c.tile_off.as<int32_t>(), c.ranges.as<int32_t>(), nc,
+                                                      c.mfp_cent.as<float>(), d, c.mfp_dab.as<double>());
             k_block_masks<<<(unsigned)n_count_groups, 256, 0, s>>>(
                 count_groups, c.tile_off.as<int32_t>(), c.ranges.as<int32_t>(), c.ctile_off.as<int32_t>(),
-                c.ctiles.as<int4>(), c.mfp_tcm.as<float>(), c.mfp_trad.as<float>(), nt, c.mfp_cent.as<float>(), d,
+                c.ctiles.as<int4>(), c.mfp_tcm.as<float>(), c.mfp_trad.as<float>(), nt, c.mfp_dab.as<double>(), d,
                 rr, (double)e2, c.mfp_moff.as<int32_t>(), c.mfp_mask.as<uint64_t>(), c.mfp_amask.as<uint64_t>(),
                 c.mfp_tacc.as<int32_t>(),
                 c.pflag.as<uint8_t>(), d_kept);
