@@ -166,7 +166,7 @@ struct Ctx {
     DevBuf offsets_d, inst_d, bounds_part, keys, vals, keys2, vals2, flag, cid, cell_of,
         cell_start, cell_key, soa, row_keys, nranges, ranges, tile_off, tiles, nn_s,
         core_s, cell_core_min, cell_has_core, core_bits, border_tiles, parent, comp_min, cell_label, label_s, rank,
-        misc, scratch, scratch2, soa_core, mfp_chi, mfp_cent, mfp_rad, mfp_trad, mfp_trho, mfp_tab, mfp_tcm, mfp_moff, mfp_mask, mfp_amask, mfp_tacc, mfp_trep, mfp_tband, mfp_mhi, mfp_mlo, mfp_mna, mfp_emask, mfp_dab, cell_c, gflag, ctile_off, ctiles, tile_tests, dense_id, dense_info, dense_tmp, fine_off, part_off, part_fine, big_id;
+        misc, scratch, scratch2, soa_core, mfp_chi, mfp_cent, mfp_rad, mfp_trad, mfp_trho, mfp_tab, mfp_tcm, mfp_moff, mfp_mask, mfp_amask, mfp_tacc, mfp_trep, mfp_tband, mfp_mhi, mfp_mlo, mfp_mna, mfp_emask, mfp_dab, mfp_colpart, cell_c, gflag, ctile_off, ctiles, tile_tests, dense_id, dense_info, dense_tmp, fine_off, part_off, part_fine, big_id;
     int64_t n_cells = 0, n_tiles = 0, n_label_tiles = 0, pair_tests = 0, n_dense = 0;
     int64_t n_rows_direct = 0;  // rows of all instances' (c2, c1) grids (eager runs)
     bool row_direct = false;          // the last grid's row table: direct (else hashed, row_mask)

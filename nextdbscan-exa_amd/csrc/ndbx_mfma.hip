@@ -78,15 +78,35 @@ constexpr int kCellTarget = 1024;  // points per partition cell, on average
 constexpr size_t kPruneBytes = (size_t)16 << 30;
 
 // ---------------------------------------------------------------- prep --
+// Column sums in two fixed-order levels (the result is deterministic):
+// workgroup g sums rows [g rows_per, (g + 1) rows_per) in order, one thread
+// per coordinate, into part[g][k]; k_colsum_total sums the workgroups'
+// parts in order.
 __global__ __launch_bounds__(256) void k_colsum(const float *__restrict__ x, int64_t n, int d,
-                                                double *__restrict__ sum) {
+                                                double *__restrict__ part) {
     const int64_t rows_per = (n + gridDim.x - 1) / gridDim.x;
     const int64_t b = blockIdx.x * rows_per, e = b + rows_per < n ? b + rows_per : n;
     for (int k = threadIdx.x; k < d; k += blockDim.x) {
         double s = 0.0;
         for (int64_t i = b; i < e; ++i) s += x[i * d + k];
-        if (e > b) atomicAdd(sum + k, s);
+        part[(int64_t)blockIdx.x * d + k] = s;
     }
+}
+// One workgroup per coordinate: thread t adds parts t, t + 256, ... in
+// order, then the threads' sums are added in a fixed tree order.
+__global__ __launch_bounds__(256) void k_colsum_total(const double *__restrict__ part, int groups, int d,
+                                                      double *__restrict__ sum) {
+    __shared__ double sp[256];
+    const int k = blockIdx.x;
+    double t = 0.0;
+    for (int g = threadIdx.x; g < groups; g += 256) t += part[(int64_t)g * d + k];
+    sp[threadIdx.x] = t;
+    __syncthreads();
+    for (int o = 128; o >= 1; o >>= 1) {
+        if (threadIdx.x < o) sp[threadIdx.x] += sp[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sum[k] = sp[0];
 }
 
 __device__ __forceinline__ __bf16 to_bf16(float f) { return (__bf16)f; }  // round to nearest even
@@ -252,6 +272,59 @@ __global__ __launch_bounds__(256) void k_centroid_prep(const float *__restrict__
     split_row<KP>(v, mu, d, hi + c * KP, lo + c * KP, na[c], rho[c]);
 }
 
+// Square distance of row p to row m, summed by a 16-lane group (gl: the
+// lane in the group): each lane takes 4 consecutive coordinates in turn
+// (16-byte loads; vec: d % 4 == 0 and both rows 16-byte aligned), or every
+// 16th coordinate, then the lanes' sums are added over the group in a fixed
+// tree order.  Every lane of the group must call it; each gets the total.
+__device__ __forceinline__ double grp_sqdist64(const float *p, const float *m, int d, bool vec, int gl) {
+    double s = 0.0;
+    if (vec) {
+        for (int k = 4 * gl; k < d; k += 64) {
+            const float4 a = *reinterpret_cast<const float4 *>(p + k), b = *reinterpret_cast<const float4 *>(m + k);
+            double t = (double)a.x - (double)b.x;
+            s += t * t;
+            t = (double)a.y - (double)b.y;
+            s += t * t;
+            t = (double)a.z - (double)b.z;
+            s += t * t;
+            t = (double)a.w - (double)b.w;
+            s += t * t;
+        }
+    } else {
+        for (int k = gl; k < d; k += 16) {
+            const double t = (double)p[k] - (double)m[k];
+            s += t * t;
+        }
+    }
+    for (int o = 8; o >= 1; o >>= 1) s += __shfl_xor(s, o, 16);
+    return s;
+}
+// The same in float32 (differences, products and sums rounded to float).
+__device__ __forceinline__ float grp_sqdist32(const float *p, const float *m, int d, bool vec, int gl) {
+    float s = 0.0f;
+    if (vec) {
+        for (int k = 4 * gl; k < d; k += 64) {
+            const float4 a = *reinterpret_cast<const float4 *>(p + k), b = *reinterpret_cast<const float4 *>(m + k);
+            float t = a.x - b.x;
+            s += t * t;
+            t = a.y - b.y;
+            s += t * t;
+            t = a.z - b.z;
+            s += t * t;
+            t = a.w - b.w;
+            s += t * t;
+        }
+    } else {
+        for (int k = gl; k < d; k += 16) {
+            const float t = p[k] - m[k];
+            s += t * t;
+        }
+    }
+    for (int o = 8; o >= 1; o >>= 1) s += __shfl_xor(s, o, 16);
+    return s;
+}
+
 // Per cell: radius (float64, max member distance to the centre point) and
 // the largest |a'| of its members.
 __global__ __launch_bounds__(256) void k_cell_radius(const float *__restrict__ xs, const float *__restrict__ rho,
@@ -266,12 +339,12 @@ __global__ __launch_bounds__(256) void k_cell_radius(const float *__restrict__ x
     double r = 0.0;
     float m = 0.0f;
     for (int32_t i = b + threadIdx.x; i < e; i += blockDim.x) {
-        double s = 0.0;
+        double s2 = 0.0;
         for (int k = 0; k < d; ++k) {
             const double df = (double)xs[(int64_t)i * d + k] - (double)cc[k];
-            s += df * df;
+            s2 += df * df;
         }
-        r = fmax(r, s);
+        r = fmax(r, s2);
         m = fmaxf(m, rho[i]);
     }
     __shared__ double sr[256];
@@ -330,19 +403,19 @@ __device__ void mf_unite(int32_t *parent, int32_t a, int32_t b) {
 // Per cell: the centroid of its points (float64 sums, rounded to float32);
 // the float32 centroid is the reference point of every radius and distance
 // bound of the block pruning.
-// One workgroup per cell: P = 256 / d point streams (point b + s + P j to
+// One workgroup per cell: P = 1024 / d point streams (point b + s + P j to
 // stream s), each summing its points per coordinate, then the streams
 // summed in order (a fixed order: the result is deterministic).
-__global__ __launch_bounds__(256) void k_cell_centroid(const float *__restrict__ x, const uint32_t *__restrict__ perm,
-                                                       const int32_t *__restrict__ cell_start, int64_t nc, int d,
-                                                       float *__restrict__ mc) {
-    __shared__ double part[256];
+__global__ __launch_bounds__(1024) void k_cell_centroid(const float *__restrict__ x, const uint32_t *__restrict__ perm,
+                                                        const int32_t *__restrict__ cell_start, int64_t nc, int d,
+                                                        float *__restrict__ mc) {
+    __shared__ double part[1024];
     const int64_t c = blockIdx.x;
     if (c >= nc) return;
     const int32_t b = cell_start[c], e = cell_start[c + 1];
-    for (int k0 = 0; k0 < d; k0 += 256) {
-        const int dk = d - k0 < 256 ? d - k0 : 256;  // coordinates of this round
-        const int P = 256 / dk;
+    for (int k0 = 0; k0 < d; k0 += 1024) {
+        const int dk = d - k0 < 1024 ? d - k0 : 1024;  // coordinates of this round
+        const int P = 1024 / dk;
         const int s = threadIdx.x / dk, k = k0 + (threadIdx.x - s * dk);
         double acc = 0.0;
         if (s < P) {
@@ -371,34 +444,29 @@ __global__ __launch_bounds__(256) void k_cell_centroid(const float *__restrict__
 __global__ __launch_bounds__(256) void k_radius_keys(const float *__restrict__ x, const uint32_t *__restrict__ perm,
                                                      const uint64_t *__restrict__ keys,
                                                      const int32_t *__restrict__ cell_of,
-                                                     const float *__restrict__ mc, int64_t n, int d,
+                                                     const float *__restrict__ mc, int64_t n, int d, bool vec,
                                                      uint64_t *__restrict__ keys2, uint32_t *__restrict__ vals2) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;  // one 16-lane group per point
+    const int gl = threadIdx.x & 15;
+    const int64_t i = i0 < n ? i0 : n - 1;
     const uint32_t p = perm[i];
-    const float *m = mc + (int64_t)cell_of[i] * d;
-    float s = 0.0f;
-    for (int k = 0; k < d; ++k) {
-        const float df = x[(int64_t)p * d + k] - m[k];
-        s += df * df;
+    const float s = grp_sqdist32(x + (int64_t)p * d, mc + (int64_t)cell_of[i] * d, d, vec, gl);
+    if (gl == 0 && i0 < n) {
+        keys2[i] = keys[i] << 32 | (uint64_t)__float_as_uint(sqrtf(s));
+        vals2[i] = p;
     }
-    keys2[i] = keys[i] << 32 | (uint64_t)__float_as_uint(sqrtf(s));
-    vals2[i] = p;
 }
 
 // Per sorted point: an upper bound of its distance to its cell's centroid
 // (float64, rounded up to float32).
 __global__ __launch_bounds__(256) void k_point_radius(const float *__restrict__ xs, const int32_t *__restrict__ cell_of,
-                                                      const float *__restrict__ mc, int64_t n, int d,
+                                                      const float *__restrict__ mc, int64_t n, int d, bool vec,
                                                       float *__restrict__ rad) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float *m = mc + (int64_t)cell_of[i] * d;
-    double s = 0.0;
-    for (int k = 0; k < d; ++k) {
-        const double df = (double)xs[i * d + k] - (double)m[k];
-        s += df * df;
-    }
+    const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4;  // one 16-lane group per point
+    const int gl = threadIdx.x & 15;
+    const int64_t i = i0 < n ? i0 : n - 1;
+    const double s = grp_sqdist64(xs + i * d, mc + (int64_t)cell_of[i] * d, d, vec, gl);
+    if (gl != 0 || i0 >= n) return;
     const double r = sqrt(s) * (1.0 + 1e-12);
     float f = (float)r;
     if ((double)f < r) f = __uint_as_float(__float_as_uint(f) + 1u);  // the next float up (f >= 0, finite)
@@ -2020,10 +2088,12 @@ void mfma_classify(Ctx &c) {
     // centre of the data (float64 sums)
     c.misc.ensure(4096);
     double *sum = reinterpret_cast<double *>(c.misc.p);
-    fill_u64(reinterpret_cast<uint64_t *>(sum), 0, d, s, c.prof);
+    constexpr int kColsumGroups = 1024;
+    c.mfp_colpart.ensure((size_t)kColsumGroups * d * 8);
     {
         Launch L(c.prof, s, "mf_colsum");
-        k_colsum<<<1024, 256, 0, s>>>(c.coords, n, d, sum);
+        k_colsum<<<kColsumGroups, 256, 0, s>>>(c.coords, n, d, c.mfp_colpart.as<double>());
+        k_colsum_total<<<(unsigned)d, 256, 0, s>>>(c.mfp_colpart.as<double>(), kColsumGroups, d, sum);
     }
     std::vector<double> h_sum(d);
     NDBX_HIP(hipMemcpyAsync(h_sum.data(), sum, d * 8, hipMemcpyDeviceToHost, s));
@@ -2100,10 +2170,13 @@ void mfma_classify(Ctx &c) {
     c.mfp_cent.ensure((size_t)nc * d * 4);
     {
         Launch L(c.prof, s, "mf_prune");
-        k_cell_centroid<<<(unsigned)nc, 256, 0, s>>>(c.coords, perm, c.cell_start.as<int32_t>(), nc, d,
-                                                    c.mfp_cent.as<float>());
-        k_radius_keys<<<nb, 256, 0, s>>>(c.coords, perm, c.keys.as<uint64_t>(), c.cell_of.as<int32_t>(),
-                                         c.mfp_cent.as<float>(), n, d, c.keys2.as<uint64_t>(), c.vals2.as<uint32_t>());
+        k_cell_centroid<<<(unsigned)nc, 1024, 0, s>>>(c.coords, perm, c.cell_start.as<int32_t>(), nc, d,
+                                                     c.mfp_cent.as<float>());
+        const bool vin = (d & 3) == 0 && ((uintptr_t)c.coords & 15) == 0;  // 16-byte rows of the input
+        k_radius_keys<<<(unsigned)ceil_div(n * 16, 256), 256, 0, s>>>(c.coords, perm, c.keys.as<uint64_t>(),
+                                                                       c.cell_of.as<int32_t>(), c.mfp_cent.as<float>(),
+                                                                       n, d, vin, c.keys2.as<uint64_t>(),
+                                                                       c.vals2.as<uint32_t>());
     }
     bool alt2 = false;
     radix_sort_u64(c.keys2.as<uint64_t>(), c.vals2.as<uint32_t>(), c.keys.as<uint64_t>(), c.vals.as<uint32_t>(), n,
@@ -2258,8 +2331,9 @@ void mfma_classify(Ctx &c) {
         c.mfp_tcm.ensure((size_t)nt * nc * 4);
         c.mfp_moff.ensure((n_count_groups + 1) * 4);
         Launch L(c.prof, s, "mf_prune");
-        k_point_radius<<<nb, 256, 0, s>>>(c.soa.as<float>(), c.cell_of.as<int32_t>(), c.mfp_cent.as<float>(), n, d,
-                                          c.mfp_rad.as<float>());
+        k_point_radius<<<(unsigned)ceil_div(n * 16, 256), 256, 0, s>>>(
+            c.soa.as<float>(), c.cell_of.as<int32_t>(), c.mfp_cent.as<float>(), n, d, (d & 3) == 0,
+            c.mfp_rad.as<float>());
         // the centroids centred and split as the points, then the bounds
         c.mfp_mhi.ensure((size_t)nc * kp * 2);
         c.mfp_mlo.ensure((size_t)nc * kp * 2);
