@@ -301,7 +301,10 @@ def bench_single(workload, steps, warmup, n=None, cpu=True, compare_plain=False,
         with ndbx.Context(x, eps, m, flags=ndbx.FP64_AUDIT) as ac:
             ac.classify()
             a = ac.audit()
-        a["scope"] = "every pair of ~512 sampled 32-row groups of the candidate cell pairs"
+        stride, offset = os.environ.get("NDBX_AUDIT_STRIDE"), int(os.environ.get("NDBX_AUDIT_OFFSET", "0"))
+        a["scope"] = ("every pair of ~512 sampled 128-row groups of the candidate cell pairs" if not stride else
+                      f"every pair of the 128-row groups {offset} + {int(stride)} j of the candidate cell pairs")
+        a["band"] = "the count pass's: R from the rows' and the column tile's largest centred norm"
         out["fp64_audit"] = a
         assert a["filter_flips"] == 0, f"filter flips: {a}"
     if graph and d <= 3:

@@ -72,6 +72,8 @@ static void open_common(Ctx &c, const float *coords, int64_t n_total, uint32_t f
     if (const char *qt = std::getenv("NDBX_QUAD_TILES")) c.quad_mode = std::atoi(qt) != 0;
     // fixed guard-band queue capacity of the MFMA count pass (tests of its overflow path)
     if (const char *bq = std::getenv("NDBX_BAND_QUEUE")) c.bq_fixed = std::max(1ll, std::atoll(bq));
+    if (const char *as = std::getenv("NDBX_AUDIT_STRIDE")) c.audit_stride = std::max(1, std::atoi(as));
+    if (const char *ao = std::getenv("NDBX_AUDIT_OFFSET")) c.audit_offset = std::max(0, std::atoi(ao));
     c.n = n_total;
     c.flags = flags;
     const size_t bytes = (size_t)n_total * (size_t)c.d * sizeof(float);

@@ -195,6 +195,8 @@ struct Ctx {
     int64_t bq_cap = 0, bq_last = 0;  // guard-band queue: capacity, last run's count
     unsigned long long bq_last64 = 0;  // (copied back after the count pass)
     int64_t bq_fixed = 0;              // > 0: capacity fixed (NDBX_BAND_QUEUE)
+    int audit_stride = 0;              // > 0: the float64 audit takes every audit_stride-th group (NDBX_AUDIT_STRIDE)
+    int audit_offset = 0;              // from this group (NDBX_AUDIT_OFFSET, below the stride)
     bool pruned = false;               // the last count pass skipped blocks by the distance bounds
 
     // audit (mfma)

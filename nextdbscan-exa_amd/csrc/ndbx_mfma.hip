@@ -914,7 +914,7 @@ struct MfmaArgs {
     uint8_t *pflag;                   // per candidate entry: the pair holds a pair within eps
     unsigned long long *audit;        // [band pairs, fp64 flips, filter flips], pass 3
     float *audit_err;                 // max |est - fp64 len|, pass 3
-    int audit_stride;
+    int audit_stride, audit_offset;  // pass 3: groups audit_offset + audit_stride j
     int union_phase;          // pass 1: 0 the cells' own pairs, 1 the other flagged pairs
     const int32_t *cell_rep;  // pass 1, phase 1: a core of the cell when all its cores share a root, else -1
     int4 *bq;          // count pass: guard-band pairs (row, column, candidate entry, own pair)
@@ -968,7 +968,7 @@ __global__ __launch_bounds__(256, (PASS == 1 && KP < 128) ? 3 : 1) void k_mfma_p
     __shared__ int32_t ss_pub[2];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     int64_t gidx = blockIdx.x;
-    if (PASS == 3) gidx *= A.audit_stride;
+    if (PASS == 3) gidx = gidx * A.audit_stride + A.audit_offset;
     if (gidx >= A.n_groups) return;
     const int4 grp = A.groups[gidx];
     const bool wave_on = wave < grp.z;
@@ -2015,7 +2015,7 @@ bool mfma_supported(int d) { return d >= 4 && d <= 128; }
 template <int PASS>
 static void launch_pass(Ctx &c, const MfmaArgs &a, int kp, const char *name) {
     int64_t blocks = a.n_groups;
-    if (PASS == 3) blocks = ceil_div(a.n_groups, a.audit_stride);
+    if (PASS == 3) blocks = ceil_div(a.n_groups - a.audit_offset, a.audit_stride);
     const unsigned g = (unsigned)blocks;
     if (g == 0) return;
     Launch L(c.prof, c.stream, name);
@@ -2464,8 +2464,18 @@ void mfma_classify(Ctx &c) {
         fill_u64(reinterpret_cast<uint64_t *>(au), 0, 4, s, c.prof);
         a.audit = au;
         a.audit_err = reinterpret_cast<float *>(au + 3);
-        a.audit_stride = (int)std::max<int64_t>(1, c.n_label_tiles / 512);
+        // every audit_stride-th group (about 512 groups, or every
+        // NDBX_AUDIT_STRIDE-th from NDBX_AUDIT_OFFSET), with the count pass's band: R from the
+        // column tile's own largest norm when the tile bounds were built
+        a.audit_stride = c.audit_stride > 0 ? c.audit_stride : (int)std::max<int64_t>(1, c.n_label_tiles / 512);
+        a.audit_offset = c.audit_offset < a.audit_stride ? c.audit_offset : 0;
+        if (c.pruned) {
+            a.tile_rho = c.mfp_trho.as<float>();
+            a.ctoff = c.ctile_off.as<int32_t>();
+        }
         launch_pass<3>(c, a, kp, "mf_audit");
+        a.tile_rho = nullptr;
+        a.ctoff = nullptr;
         unsigned long long h[4];
         NDBX_HIP(hipMemcpyAsync(h, au, 32, hipMemcpyDeviceToHost, s));
         c.sync();
