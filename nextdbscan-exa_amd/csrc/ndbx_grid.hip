@@ -459,10 +459,6 @@ __device__ __forceinline__ uint64_t mix64(uint64_t v) {
 
 constexpr uint64_t kEmpty = ~0ull;
 
-// Stencil rows per cell, (2 reach + 1)^(D-1) at reach 2.
-template <int D>
-constexpr int kStencilRows = D == 3 ? 25 : D == 2 ? 5 : 1;
-
 // Cell coordinates (c0, c1, c2) of every cell from its key, as three planes
 // of cell_c, decoded once per cell (32-bit division when the key's linear
 // index and the grid sides fit, as they do unless an instance has 2^32 cells).

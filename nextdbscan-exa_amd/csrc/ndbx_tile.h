@@ -6,6 +6,10 @@
 
 namespace ndbx {
 
+// Stencil rows per cell, (2 reach + 1)^(D-1) at reach 2.
+template <int D>
+constexpr int kStencilRows = D == 3 ? 25 : D == 2 ? 5 : 1;
+
 constexpr int kStage = 256;  // candidates staged in LDS per wave
 
 __device__ __forceinline__ float wave_min(float v) {
