@@ -111,6 +111,16 @@ __global__ __launch_bounds__(256) void k_colsum_total(const double *__restrict__
 
 __device__ __forceinline__ __bf16 to_bf16(float f) { return (__bf16)f; }  // round to nearest even
 
+// Partition centres: row j is point j n / k_cent (one launch; a copy per
+// centre queued ~1,000 copies a run).
+__global__ __launch_bounds__(256) void k_pick_centres(const float *__restrict__ x, int64_t n, int d, int64_t k_cent,
+                                                      float *__restrict__ cent) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= k_cent * d) return;
+    const int64_t j = t / d, k = t - j * d;
+    cent[t] = x[(j * n / k_cent) * d + k];
+}
+
 // The centres' bf16 rows (centred on mu, padded to KP, the padding centres
 // up to a multiple of 32 all zero) and their squared norms (+inf for the
 // padding, which is then never the nearest).
@@ -2106,8 +2116,10 @@ void mfma_classify(Ctx &c) {
     float *cent = c.flag.as<float>();
     float *mu = cent + k_cent * d;
     NDBX_HIP(hipMemcpyAsync(mu, h_mu.data(), d * 4, hipMemcpyHostToDevice, s));
-    for (int64_t j = 0; j < k_cent; ++j)
-        NDBX_HIP(hipMemcpyAsync(cent + j * d, c.coords + (j * n / k_cent) * d, d * 4, hipMemcpyDeviceToDevice, s));
+    {
+        Launch L(c.prof, s, "mf_assign");
+        k_pick_centres<<<(unsigned)ceil_div(k_cent * d, 256), 256, 0, s>>>(c.coords, n, d, k_cent, cent);
+    }
     c.keys.ensure(n * 8);
     c.vals.ensure(n * 4);
     // the hi rows are read 32 at a time from any row up to n: 32 rows of
