@@ -1008,6 +1008,14 @@ __global__ __launch_bounds__(256, (PASS == 1 && KP < 128) ? 3 : 1) void k_mfma_p
     float na_r[16];
     int32_t acc_i[16], rp_r[16], root_r[16];
     uint32_t rlive = 0;
+    // PASS 1: each live row's root found once, by lane `row` (the 32 lanes
+    // of a half hold the same rows: finding them in every lane repeated each
+    // path-halving write 32 times), then handed to the lanes that hold it
+    int32_t my_root = -1;
+    if (PASS == 1 && half == 0 && col < nrows) {
+        const int64_t p = row_point(col);
+        if (A.core_s[p]) my_root = mf_find(A.parent, (int32_t)p);
+    }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int tr = 8 * (r >> 2) + 4 * half + (r & 3);
@@ -1020,7 +1028,12 @@ __global__ __launch_bounds__(256, (PASS == 1 && KP < 128) ? 3 : 1) void k_mfma_p
         rlive |= (uint32_t)live << r;
         na_r[r] = live ? A.na[p] : INF;
         acc_i[r] = PASS == 2 ? NDBX_NO_CLUSTER : 0;
-        root_r[r] = (PASS == 1 && live) ? mf_find(A.parent, (int32_t)p) : -1;
+        if (PASS == 1) {
+            const int32_t rt = __shfl(my_root, tr, 64);
+            root_r[r] = live ? rt : -1;
+        } else {
+            root_r[r] = -1;
+        }
     }
     if (PASS == 1 || PASS == 2) {  // whole workgroup leaves when no wave has work
         if (threadIdx.x == 0) any_work = 0;
