@@ -877,21 +877,33 @@ __device__ __forceinline__ bool exact_pred(const float *__restrict__ p, const fl
 }
 
 // The same predicate, coordinates read 16 bytes at a time (d % 4 == 0, rows
-// 16-byte aligned); the sum is still taken in ascending k.
+// 16-byte aligned), 32 of each row per round trip: the loads of a chunk are
+// all issued before its sum, which is still taken in ascending k.
 __device__ __forceinline__ bool exact_pred4(const float *__restrict__ p, const float *__restrict__ q, int d,
                                             float e2) {
     const float4 *p4 = reinterpret_cast<const float4 *>(p), *q4 = reinterpret_cast<const float4 *>(q);
+    const int n4 = d >> 2;
     float len = 0.0f;
-    for (int k = 0; k < d / 4; ++k) {
-        const float4 a = p4[k], b = q4[k];
-        float df = a.x - b.x;
-        len = __fadd_rn(len, __fmul_rn(df, df));
-        df = a.y - b.y;
-        len = __fadd_rn(len, __fmul_rn(df, df));
-        df = a.z - b.z;
-        len = __fadd_rn(len, __fmul_rn(df, df));
-        df = a.w - b.w;
-        len = __fadd_rn(len, __fmul_rn(df, df));
+    for (int k0 = 0; k0 < n4; k0 += 8) {
+        float4 a[8], b[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (k0 + j < n4) {
+                a[j] = p4[k0 + j];
+                b[j] = q4[k0 + j];
+            }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (k0 + j < n4) {
+                float df = a[j].x - b[j].x;
+                len = __fadd_rn(len, __fmul_rn(df, df));
+                df = a[j].y - b[j].y;
+                len = __fadd_rn(len, __fmul_rn(df, df));
+                df = a[j].z - b[j].z;
+                len = __fadd_rn(len, __fmul_rn(df, df));
+                df = a[j].w - b[j].w;
+                len = __fadd_rn(len, __fmul_rn(df, df));
+            }
     }
     return len <= e2;
 }
