@@ -84,12 +84,39 @@ constexpr size_t kPruneBytes = (size_t)16 << 30;
 // parts in order.
 __global__ __launch_bounds__(256) void k_colsum(const float *__restrict__ x, int64_t n, int d,
                                                 double *__restrict__ part) {
+    __shared__ double sp[256];
     const int64_t rows_per = (n + gridDim.x - 1) / gridDim.x;
     const int64_t b = blockIdx.x * rows_per, e = b + rows_per < n ? b + rows_per : n;
-    for (int k = threadIdx.x; k < d; k += blockDim.x) {
-        double s = 0.0;
-        for (int64_t i = b; i < e; ++i) s += x[i * d + k];
-        part[(int64_t)blockIdx.x * d + k] = s;
+    if (d > 256) {
+        for (int k = threadIdx.x; k < d; k += blockDim.x) {
+            double s = 0.0;
+            for (int64_t i = b; i < e; ++i) s += x[i * d + k];
+            part[(int64_t)blockIdx.x * d + k] = s;
+        }
+        return;
+    }
+    // P interleaved row streams of the group's rows, each summed in order,
+    // then the streams' sums added in stream order
+    const int P = 256 / d, k = threadIdx.x % d, st = threadIdx.x / d;
+    double s = 0.0;
+    if (st < P) {
+        int64_t i = b + st;
+        for (; i + 3 * P < e; i += 4 * P) {
+            const float v0 = x[i * d + k], v1 = x[(i + P) * d + k], v2 = x[(i + 2 * P) * d + k],
+                        v3 = x[(i + 3 * P) * d + k];
+            s += v0;
+            s += v1;
+            s += v2;
+            s += v3;
+        }
+        for (; i < e; i += P) s += x[i * d + k];
+    }
+    sp[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < d) {
+        double t = 0.0;
+        for (int q = 0; q < P; ++q) t += sp[q * d + threadIdx.x];
+        part[(int64_t)blockIdx.x * d + threadIdx.x] = t;
     }
 }
 // One workgroup per coordinate: thread t adds parts t, t + 256, ... in
@@ -348,11 +375,38 @@ __global__ __launch_bounds__(256) void k_cell_radius(const float *__restrict__ x
     const float *cc = cent + (int64_t)cell_key[c] * d;
     double r = 0.0;
     float m = 0.0f;
+    const bool v4 = (d & 3) == 0;  // rows and the centroid 16-byte aligned
     for (int32_t i = b + threadIdx.x; i < e; i += blockDim.x) {
         double s2 = 0.0;
-        for (int k = 0; k < d; ++k) {
-            const double df = (double)xs[(int64_t)i * d + k] - (double)cc[k];
-            s2 += df * df;
+        if (v4) {
+            // 32 coordinates of the row per round trip, summed in ascending k
+            const float4 *x4 = reinterpret_cast<const float4 *>(xs + (int64_t)i * d);
+            const float4 *c4 = reinterpret_cast<const float4 *>(cc);
+            const int n4 = d >> 2;
+            for (int k0 = 0; k0 < n4; k0 += 8) {
+                float4 a[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (k0 + j < n4) a[j] = x4[k0 + j];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (k0 + j < n4) {
+                        const float4 cv = c4[k0 + j];
+                        double df = (double)a[j].x - (double)cv.x;
+                        s2 += df * df;
+                        df = (double)a[j].y - (double)cv.y;
+                        s2 += df * df;
+                        df = (double)a[j].z - (double)cv.z;
+                        s2 += df * df;
+                        df = (double)a[j].w - (double)cv.w;
+                        s2 += df * df;
+                    }
+            }
+        } else {
+            for (int k = 0; k < d; ++k) {
+                const double df = (double)xs[(int64_t)i * d + k] - (double)cc[k];
+                s2 += df * df;
+            }
         }
         r = fmax(r, s2);
         m = fmaxf(m, rho[i]);
