@@ -36,10 +36,28 @@ __global__ __launch_bounds__(256) void k_bounds(const float *__restrict__ x, int
     const int64_t b = chunk_begin[blockIdx.x], e = chunk_end[blockIdx.x];
     float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
     int32_t special = 0;
+    // rows of 4k coordinates at a 16-byte aligned base: 32 coordinates of a
+    // row per round trip
+    const bool v4 = (d & 3) == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0;
     for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
         const float *p = x + i * d;
         bool sp = p[0] == FLT_MAX;
-        for (int k = 0; k < d; ++k) sp |= !isfinite(p[k]);
+        if (v4) {
+            const float4 *p4 = reinterpret_cast<const float4 *>(p);
+            const int n4 = d >> 2;
+            for (int k0 = 0; k0 < n4; k0 += 8) {
+                float4 a[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (k0 + j < n4) a[j] = p4[k0 + j];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (k0 + j < n4)
+                        sp |= !isfinite(a[j].x) || !isfinite(a[j].y) || !isfinite(a[j].z) || !isfinite(a[j].w);
+            }
+        } else {
+            for (int k = 0; k < d; ++k) sp |= !isfinite(p[k]);
+        }
         if (sp) {
             ++special;
             continue;

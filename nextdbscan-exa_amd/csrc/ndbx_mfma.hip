@@ -738,9 +738,33 @@ __global__ __launch_bounds__(256) void k_pair_cdist(const int32_t *__restrict__ 
     for (int32_t i = cb + threadIdx.x; i < ce; i += blockDim.x) {
         const float *mB = mc + (int64_t)cand[i] * d;
         double dd = 0.0;
-        for (int k = 0; k < d; ++k) {
-            const double df = (double)mA[k] - (double)mB[k];
-            dd += df * df;
+        if ((d & 3) == 0) {  // 32 coordinates of B per round trip, ascending k
+            const float4 *a4 = reinterpret_cast<const float4 *>(mA), *b4 = reinterpret_cast<const float4 *>(mB);
+            const int n4 = d >> 2;
+            for (int k0 = 0; k0 < n4; k0 += 8) {
+                float4 bv[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (k0 + j < n4) bv[j] = b4[k0 + j];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (k0 + j < n4) {
+                        const float4 av = a4[k0 + j];
+                        double df = (double)av.x - (double)bv[j].x;
+                        dd += df * df;
+                        df = (double)av.y - (double)bv[j].y;
+                        dd += df * df;
+                        df = (double)av.z - (double)bv[j].z;
+                        dd += df * df;
+                        df = (double)av.w - (double)bv[j].w;
+                        dd += df * df;
+                    }
+            }
+        } else {
+            for (int k = 0; k < d; ++k) {
+                const double df = (double)mA[k] - (double)mB[k];
+                dd += df * df;
+            }
         }
         dab[i] = sqrt(dd) * (1.0 + 1e-12);
     }
