@@ -449,17 +449,20 @@ __device__ int32_t mf_find(int32_t *parent, int32_t x) {
         x = gp;
     }
 }
-__device__ void mf_unite(int32_t *parent, int32_t a, int32_t b) {
+// Returns the root of the united set as this call left it (a later union
+// may hang it lower); a may be any member of its set, such as a root
+// returned before, which shortens its find.
+__device__ int32_t mf_unite(int32_t *parent, int32_t a, int32_t b) {
     for (;;) {
         a = mf_find(parent, a);
         b = mf_find(parent, b);
-        if (a == b) return;
+        if (a == b) return a;
         if (a > b) {
             const int32_t t = a;
             a = b;
             b = t;
         }
-        if (atomicCAS(parent + b, b, a) == b) return;
+        if (atomicCAS(parent + b, b, a) == b) return a;
     }
 }
 
@@ -822,6 +825,7 @@ __global__ __launch_bounds__(256) void k_allin_union(const int4 *__restrict__ gr
     __syncthreads();
     const int32_t cb = cand_off[grp.x], ce = cand_off[grp.x + 1];
     bool any = false;
+    int32_t ra = r0;  // a member of the group's set: the last root seen
     for (int32_t i = cb + threadIdx.x; i < ce; i += blockDim.x) {
         uint64_t m = amasks[moff[g] + (i - cb)];
         if (!m) continue;
@@ -831,7 +835,7 @@ __global__ __launch_bounds__(256) void k_allin_union(const int4 *__restrict__ gr
             m &= m - 1;
             const int32_t rj = rep[J];
             if (rj < 0) continue;
-            mf_unite(parent, r0, rj);
+            ra = mf_unite(parent, ra, rj);
             tflag[J] = 1;
             any = true;
         }
@@ -1439,8 +1443,7 @@ __global__ __launch_bounds__(256, (PASS == 1 && KP < 128) ? 3 : 1) void k_mfma_p
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
                     if (!((in_m >> r) & 1) || (cc == cell && bcol <= rp_r[r]) || root_r[r] == aux) continue;
-                    mf_unite(A.parent, rp_r[r], bcol);
-                    root_r[r] = mf_find(A.parent, rp_r[r]);
+                    root_r[r] = mf_unite(A.parent, root_r[r] >= 0 ? root_r[r] : rp_r[r], bcol);
                 }
             }
             if (PASS == 2 && in_m) {
