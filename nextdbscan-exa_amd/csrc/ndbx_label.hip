@@ -777,16 +777,24 @@ __global__ __launch_bounds__(256) void k_border_pts(const int32_t *__restrict__ 
             if (LB >= best) continue;  // NO_CLUSTER (no core) included
             const int b1 = cell_start[B + 1];
             bool hit = false;
-            for (int j = cell_start[B]; j < b1 && !hit; ++j) {
-                float p[D];
-                load_core<D>(core_pts, j, p);
-                float len = 0.0f;
+            // four points per round trip (a hit is any core within eps, so
+            // the order of the tests does not matter)
+            for (int j = cell_start[B]; j < b1 && !hit; j += 4) {
+                float p[4][D];
 #pragma unroll
-                for (int k = 0; k < D; ++k) {
-                    const float df = qx[k] - p[k];
-                    len = __fadd_rn(len, __fmul_rn(df, df));
-                }
-                hit = len <= e2;
+                for (int u = 0; u < 4; ++u)
+                    if (j + u < b1) load_core<D>(core_pts, j + u, p[u]);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (j + u < b1) {
+                        float len = 0.0f;
+#pragma unroll
+                        for (int k = 0; k < D; ++k) {
+                            const float df = qx[k] - p[u][k];
+                            len = __fadd_rn(len, __fmul_rn(df, df));
+                        }
+                        hit = hit || len <= e2;
+                    }
             }
             if (hit) best = LB;
         }
